@@ -1,0 +1,216 @@
+/* oceano_dg.h -- C ABI of the MI355X shallow-water DG stage operator (liboceano_dg.so).
+ *
+ * Plain pointers and sizes only: no HIP, torch or deal.II types cross this boundary.
+ * Device buffers are hipMalloc'ed by the caller; `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream).  Every call returns an int status (OC_OK = 0); the message of
+ * the last failure on the calling thread is oc_last_error().  No exception crosses the ABI.
+ * A handle is not thread-safe and belongs to the GPU that was current at oc_dg_create.
+ *
+ * Vector storage.  The operator owns the storage order of its DoF vectors, as deal.II's
+ * MatrixFree::initialize_dof_vector does (OceanDG.h:498-504): cells are grouped in
+ * batches of OC_CELL_BATCH consecutive cells and, inside a batch, the entry of
+ * (component, node) for the batch's lane-th cell sits at
+ *   ((batch * n_components + component) * dofs_per_cell + node) * OC_CELL_BATCH + lane.
+ * `node` is lexicographic over the (p+1)^2 Gauss-Lobatto nodes (x fastest), the
+ * FE_DGQ(p) numbering.  oc_dg_pack / oc_dg_unpack convert from / to the cell-major
+ * order (cell, component, node) used by deal.II's FESystem(FE_DGQ(p), c) DoF numbering.
+ * The time integrator only copies and combines whole vectors, so it never depends on it.
+ */
+#ifndef OCEANO_DG_H
+#define OCEANO_DG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OC_CELL_BATCH 64
+
+/* status codes */
+#define OC_OK 0
+#define OC_ERR_ARGUMENT 1      /* bad argument (null pointer, size, degree, stage index)    */
+#define OC_ERR_MESH 2          /* inconsistent topology (face not covered, 2:1 violated)   */
+#define OC_ERR_BOUNDARY 3      /* boundary id without a condition: OceanDG.h:1199-1203     */
+#define OC_ERR_HIP 4           /* a HIP runtime call failed                                 */
+#define OC_ERR_NONFINITE 5     /* non-finite entry found by oc_dg_count_nonfinite callers   */
+#define OC_ERR_UNSUPPORTED 6   /* configuration the kernels do not implement                */
+
+const char *oc_last_error(void);
+
+/* ------------------------------------------------------------------------------------
+ * Boundary-condition kinds.  Replaces ICBC::BcBase's six boundary maps
+ * (IcbcBase.h:65-77): each boundary id carries exactly one kind (IcbcBase.h:146-260).
+ * ---------------------------------------------------------------------------------- */
+#define OC_BC_WALL 1
+#define OC_BC_SUPERCRITICAL_INFLOW 2
+#define OC_BC_SUPERCRITICAL_OUTFLOW 3
+#define OC_BC_HEIGHT_INFLOW 4
+#define OC_BC_DISCHARGE_INFLOW 5
+#define OC_BC_ABSORBING_OUTFLOW 6
+
+/* ------------------------------------------------------------------------------------
+ * Mesh builder (setup path).  Replaces GridIn::read_msh + Triangulation::refine_global
+ * + the MatrixFree face setup the reference reaches through main.cc:617-663 and
+ * OceanDG.h:438-470, for the axis-aligned quadrilateral meshes of the test cases.
+ * ---------------------------------------------------------------------------------- */
+typedef struct oc_mesh oc_mesh;
+
+/* nx x ny cells on [x0,x1]x[y0,y1]; boundary_id = {x0 side, x1 side, y0 side, y1 side}
+ * (deal.II face numbers 0..3); a periodic direction ignores its two ids. */
+int oc_mesh_create_rectangle(double x0, double x1, double y0, double y1, int32_t nx,
+                             int32_t ny, int32_t periodic_x, int32_t periodic_y,
+                             const int32_t boundary_id[4], oc_mesh **mesh);
+/* split every active cell into four (Triangulation::refine_global) */
+int oc_mesh_refine_global(oc_mesh *mesh, int32_t times);
+/* split round(fraction * n_active) active cells chosen by a Fisher-Yates shuffle driven
+ * by std::mt19937_64(seed) with index = engine() % (i + 1); fails if 2:1 is violated */
+int oc_mesh_refine_fraction(oc_mesh *mesh, double fraction, uint64_t seed);
+/* info = {n_cells, n_inner_faces, n_boundary_faces, n_hanging_subfaces} */
+int oc_mesh_info(const oc_mesh *mesh, int64_t info[4]);
+/* cell_geom [n_cells][4] = x0, y0, hx, hy; cell_level [n_cells];
+ * inner [n_inner][6] = cell_m, face_m, subface_m, cell_p, face_p, subface_p
+ *   (subface = -1 for a whole face, 0/1 = lower/upper half of a coarse face);
+ * boundary [n_bdry][3] = cell, face, boundary_id.  Any pointer may be NULL. */
+int oc_mesh_export(const oc_mesh *mesh, double *cell_geom, int32_t *cell_level,
+                   int32_t *inner, int32_t *boundary);
+void oc_mesh_destroy(oc_mesh *mesh);
+
+/* ------------------------------------------------------------------------------------
+ * Stage operator.  Replaces SpaceDiscretization::OceanoOperator (OceanDG.h:157-380).
+ * ---------------------------------------------------------------------------------- */
+typedef struct oc_dg oc_dg;
+
+typedef struct
+{
+  int32_t n_cells;
+  const double *cell_geom;   /* [n_cells][4] x0, y0, hx, hy (axis-aligned cells)         */
+  int32_t n_inner;
+  const int32_t *inner;      /* [n_inner][6] as oc_mesh_export                            */
+  int32_t n_boundary;
+  const int32_t *boundary;   /* [n_boundary][3] as oc_mesh_export                         */
+} oc_mesh_desc;
+
+#define OC_PRECISION_FP64 0  /* plain FP64: the reference arithmetic                       */
+#define OC_PRECISION_MIXED 1 /* FP32 cell/face integrals, FP64 state, depth and bathymetry */
+
+typedef struct
+{
+  int32_t degree;               /* fe_degree, 1..4 (main.cc:213)                           */
+  double g;                     /* "g" (ShallowWater.h:294)                                */
+  double threshold_wetdry;      /* "Threshold_for_wetdry", cu (ShallowWater.h:295-296)     */
+  double viscosity;             /* "horizontal_viscosity" (ViscosityCoefficient.h:68)      */
+  int32_t max_iteration_height; /* 15 in the reference (main.cc:226)                       */
+  int32_t precision;            /* OC_PRECISION_*                                          */
+  int32_t n_boundary_kinds;     /* entries of the two arrays below                         */
+  const int32_t *boundary_ids;  /* boundary ids that carry a condition                     */
+  const int32_t *boundary_kinds;/* OC_BC_* per id (IcbcBase.h:146-260)                     */
+} oc_params;
+
+/* Replaces OceanoOperator::OceanoOperator + reinit (OceanDG.h:415-470). */
+int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *params, oc_dg **op);
+int oc_dg_destroy(oc_dg *op);
+
+/* Static data at quadrature points, the five CellDataStorage stores of
+ * OceanoOperator::initialize_data_at_quadrature (OceanDG.h:498-569):
+ *   OC_STORE_CELL_GAUSS   [n_cells][nq*nq]   points; 5 values each: z_b, friction,
+ *                                             tau_x, tau_y, coriolis  (problem_data)
+ *   OC_STORE_CELL_LOBATTO [n_cells][ng*ng]   points; 1 value: z_b
+ *   OC_STORE_CELL_COLLOC  [n_cells][n1*n1]   points; 2 values: z_b, friction
+ *   OC_STORE_FACE         [n_inner][ng][2]   points x -+ 1e-12 n (minus, plus side); 1 value
+ *   OC_STORE_BOUNDARY     [n_bdry][nq]       points; 1 value: z_b
+ * with nq = floor(1.5p)+1, ng = p+2, n1 = p+1 and points lexicographic (x fastest).
+ * oc_dg_quadrature_points writes 2 doubles (x, y) per point to host memory; the caller
+ * evaluates its functions there and hands the values to oc_dg_set_static_data (host). */
+#define OC_STORE_CELL_GAUSS 0
+#define OC_STORE_CELL_LOBATTO 1
+#define OC_STORE_CELL_COLLOC 2
+#define OC_STORE_FACE 3
+#define OC_STORE_BOUNDARY 4
+int oc_dg_n_points(const oc_dg *op, int32_t store, int64_t *n_points);
+int oc_dg_quadrature_points(const oc_dg *op, int32_t store, double *xy);
+int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values);
+
+/* Boundary data at the boundary quadrature points, [n_bdry][nq][3]: the values of the
+ * boundary-condition Function at its current time (OceanDG.h:1146-1184 evaluate them per
+ * quadrature point; here the host tabulates them, per stage when they depend on time).
+ * Component 0 = free surface, 1..2 = velocity (supercritical inflow, absorbing outflow);
+ * component 1 = the discharge magnitude for OC_BC_DISCHARGE_INFLOW.  Host memory. */
+int oc_dg_set_boundary_values(oc_dg *op, const double *values);
+
+/* Geometry queries (host memory): cells [n_cells][4], degree and sizes {n_cells,
+ * n_cells_padded, dofs_per_cell, nq, ng, n_inner, n_bdry}. */
+int oc_dg_sizes(const oc_dg *op, int64_t sizes[7]);
+/* number of doubles of an operator vector with n_components components (padded) */
+int oc_dg_vector_length(const oc_dg *op, int32_t n_components, int64_t *length);
+/* device-to-device conversions between cell-major order and the operator order */
+int oc_dg_pack(oc_dg *op, int32_t n_components, const double *cell_major, double *op_order,
+               void *stream);
+int oc_dg_unpack(oc_dg *op, int32_t n_components, const double *op_order, double *cell_major,
+                 void *stream);
+
+/* One explicit Runge-Kutta stage of the hydrodynamic operator.  Replaces
+ * OceanoOperator::perform_stage_hydro (OceanDG.h:182-194, body 1782-1912) as called by
+ * ExplicitRungeKuttaIntegrator::perform_time_step (ExplicitRungeKuttaIntegrator.h:212-271).
+ *   factor[0..stage]   dt-scaled Butcher row (b for the last stage), n_factor = stage+1
+ *   ri_h, ri_q, ri_vel current_ri = {zeta_r, q_r, (tracer), velocity}
+ *   ki_h[0..n_ki-1]    vec_ki_height: [0] scratch (front), [stage+1] <- L_zeta (not inverted)
+ *   ki_q[0..n_ki-1]    vec_ki_discharge: [0] scratch (L_q), [stage+1] <- M^-1 L_q
+ *   w_h, w_q           solution vectors; updated in place at the last stage
+ *   r_h, r_q           next_ri vectors; written at the other stages
+ * ki_h / ki_q are host arrays of n_ki device pointers; n_ki = n_stages + 1 and the last
+ * stage is stage == n_ki - 2 (OceanDG.h:1827).  The scratch entries ki_h[0], ki_q[0] are
+ * written only if `write_scratch` is nonzero (the time integrator never reads them).
+ * ri_vel is read only when the viscosity is nonzero (it multiplies the only use). */
+int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const double *factor,
+                              uint32_t n_factor, const double *ri_h, const double *ri_q,
+                              const double *ri_vel, double *const *ki_h, double *const *ki_q,
+                              uint32_t n_ki, double *w_h, double *w_q, double *r_h,
+                              double *r_q, int32_t write_scratch, void *stream);
+
+/* dst = M^-1 L(src): step-67's OceanoOperator::apply (declared OceanDG.h:178-180; its
+ * body in the reference is dead code).  Both fields use the exact tensor-product inverse
+ * of the mass matrix (affine cells); L is the spatial operator of perform_stage_hydro. */
+int oc_dg_apply(oc_dg *op, double time, const double *src_h, const double *src_q,
+                const double *src_vel, double *dst_h, double *dst_q, void *stream);
+
+/* L_zeta and L_q without inverse mass (the two MatrixFree::loop calls, OceanDG.h:1805-1823) */
+int oc_dg_rhs(oc_dg *op, const double *src_h, const double *src_q, const double *src_vel,
+              double *dst_h, double *dst_q, void *stream);
+
+/* Nodal coefficients from values at the OC_STORE_CELL_COLLOC points (Gauss(p+1)^2 per
+ * cell): transform_from_q_points_to_basis of deal.II's CellwiseInverseMassMatrix, as used
+ * by OceanoOperator::project_hydro (OceanDG.h:2242-2352) and evaluate_velocity_field
+ * (OceanDG.h:2600-2662).  values [n_cells][n1*n1][n_comp] -> coeffs [n_cells][n_comp][n1*n1]
+ * (cell-major).  Host memory, no GPU work. */
+int oc_dg_project_collocation(const oc_dg *op, int32_t n_comp, const double *values,
+                              double *coeffs);
+
+/* Count non-finite entries of a device vector of `n` doubles into *count (host).  The
+ * reference has no such check (main.cc:1527-1529 only mentions blow-up); hosts use it
+ * to turn a NaN state into OC_ERR_NONFINITE. Synchronises the stream. */
+int oc_dg_count_nonfinite(oc_dg *op, const double *x, int64_t n, int64_t *count, void *stream);
+
+/* Sum of Jacobi sweeps executed by the depth-mass solve in the last stage call, over all
+ * cells (the k loop of OceanDG.h:1443).  Synchronises the stream. */
+int oc_dg_last_sweeps(oc_dg *op, int64_t *sweeps, void *stream);
+
+/* ------------------------------------------------------------------------------------
+ * Test-case functions (ICBC layer).  Host memory; evaluate at n points xy[n][2].
+ * ---------------------------------------------------------------------------------- */
+/* Travelling cos^4 vortex of Icbc_ShallowWaterVortex.h:139-226 (regularity p = 2),
+ * parametrised: prm = {h0, u_inf, v_inf, amplitude (h0 - h_min), radius, xc, yc, g,
+ * period_x (0 = none), period_y}; out[n][3] = depth, u, v at time t. */
+int oc_icbc_vortex(const double prm[10], double t, int64_t n, const double *xy, double *out);
+/* Channel bed with a parabolic bump b(x) = 0.2 - 0.05 (x-10)^2 on 8 < x < 12 plus seeded
+ * Gaussian sub-cell bumps; returns problem_data rows out[n][5] = {z_b = -b, manning, 0,0,0}.
+ * prm = {manning, n_bumps(-1 = draw from seed), amplitude_min, amplitude_max, width_min,
+ * width_max, x_min, x_max}. */
+int oc_icbc_channel(const double prm[8], uint64_t seed, int64_t n, const double *xy,
+                    double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

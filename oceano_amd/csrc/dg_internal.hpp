@@ -1,0 +1,108 @@
+// dg_internal.hpp -- operator state shared by the host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "dg_kernels.hpp"
+#include "oceano_dg.h"
+
+namespace oceano {
+void set_error(const std::string &msg);
+}
+
+struct oc_dg
+{
+  using OpArgs = oceano::OpArgs;
+  int P, n1, nd, nq, ng;
+  int N, NB;
+  int precision;
+  double g, cu4, nu;
+  int max_it;
+  std::vector<double> geom;
+  std::vector<int32_t> inner;     // canonical (minus side owns face 1 or 3)
+  std::vector<char> swapped;      // inner face listed from its plus side by the caller
+  std::vector<int32_t> inner_in;  // as given
+  std::vector<int32_t> bdry;
+  std::map<int, int> bc_of_id;
+  // device
+  double *d_hxy = nullptr;
+  int *d_nbr = nullptr, *d_fid = nullptr, *d_finfo = nullptr, *d_bkind = nullptr;
+  double *d_cell0[5] = {};
+  double c_cell0[5] = {};
+  double *d_cell1 = nullptr;
+  double c_cell1 = 0;
+  double *d_face = nullptr;
+  double c_face = 0;
+  double *d_bz = nullptr;
+  double c_bz = 0;
+  double *d_bvals = nullptr;
+  unsigned long long *d_sweeps = nullptr;
+  unsigned long long *d_count = nullptr;
+  std::vector<double> cell2_host;
+  int xcd_swizzle = 1;
+
+  OpArgs args() const
+  {
+    OpArgs o;
+    o.n_cells = N;
+    o.n_batches = NB;
+    o.hxy = d_hxy;
+    o.nbr = d_nbr;
+    o.fid = d_fid;
+    o.finfo = d_finfo;
+    for (int f = 0; f < 5; ++f)
+      o.cell0[f] = {d_cell0[f], c_cell0[f]};
+    o.cell1 = {d_cell1, c_cell1};
+    o.facez = {d_face, c_face};
+    o.bdryz = {d_bz, c_bz};
+    o.bvals = d_bvals;
+    o.bkind = d_bkind;
+    o.g = g;
+    o.cu4 = cu4;
+    o.nu = nu;
+    o.has_friction = (d_cell0[1] != nullptr) || (c_cell0[1] != 0.0);
+    o.has_wind = (d_cell0[2] != nullptr) || (c_cell0[2] != 0.0) || (d_cell0[3] != nullptr) ||
+                 (c_cell0[3] != 0.0);
+    o.has_coriolis = (d_cell0[4] != nullptr) || (c_cell0[4] != 0.0);
+    o.max_it = max_it;
+    o.xcd_swizzle = xcd_swizzle;
+    return o;
+  }
+};
+
+
+namespace oceano {
+
+#define HIP_TRY(call)                                                                       \
+  do                                                                                        \
+    {                                                                                       \
+      hipError_t e_ = (call);                                                               \
+      if (e_ != hipSuccess)                                                                 \
+        {                                                                                   \
+          set_error(std::string(#call) + ": " + hipGetErrorString(e_));                     \
+          return OC_ERR_HIP;                                                                \
+        }                                                                                   \
+    }                                                                                       \
+  while (0)
+
+// per-degree translation units (dg_stage_p<P>.hip)
+#define OC_DECLARE_DEGREE(P)                                                                \
+  int upload_tables_d##P(const DegreeTables *t);                                            \
+  int stage_d##P(oc_dg *op, const OpArgs &O, uint32_t stage, const double *factor,          \
+                 const double *ri_h, const double *ri_q, const double *ri_v,                \
+                 double *const *ki_h, double *const *ki_q, uint32_t n_ki, double *w_h,      \
+                 double *w_q, double *r_h, double *r_q, int write_scratch, hipStream_t s);  \
+  int rhs_apply_d##P(oc_dg *op, const OpArgs &O, bool invert, const double *src_h,          \
+                     const double *src_q, const double *src_v, double *dst_h,              \
+                     double *dst_q, hipStream_t s);
+OC_DECLARE_DEGREE(1)
+OC_DECLARE_DEGREE(2)
+OC_DECLARE_DEGREE(3)
+OC_DECLARE_DEGREE(4)
+
+}  // namespace oceano

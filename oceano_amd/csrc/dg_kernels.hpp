@@ -1,0 +1,112 @@
+// dg_kernels.hpp -- gfx950 kernels of one explicit RK stage of the shallow-water DG operator.
+//
+// Mapping: one thread per cell, one wave64 per batch of 64 consecutive cells.  Vectors
+// and static data are stored lane-batched ([batch][...][lane]), so every load/store of a
+// cell's own data is a fully coalesced 512-byte wave access; the 1-D basis tables live in
+// __constant__ memory and are read with scalar loads (uniform across the wave).
+// Tensor-product contractions are sum-factorised row by row (one quadrature row b at a
+// time), which keeps the per-thread live set at O((p+1)^2) values.
+//
+// Faces are evaluated cell-centrically: each cell evaluates the flux on each of its
+// faces (both traces, identical arithmetic and argument order on both sides, so both
+// cells see bit-identical flux values) and integrates it against its own test functions.
+// There is no face buffer and no atomic; accumulation order is fixed.
+#pragma once
+
+#include "dg_physics.hpp"
+#include "oceano_dg.h"
+
+namespace oceano {
+
+constexpr int CB = OC_CELL_BATCH;
+constexpr int MAXN1 = 5, MAXNQ = 7, MAXNG = 6, MAXKI = 9;
+
+struct DegreeTables
+{
+  double Sq[MAXNQ][MAXN1];      // l_i at Gauss(nq) points
+  double Dq[MAXNQ][MAXN1];      // l_i' at Gauss(nq) points
+  double wq[MAXNQ];
+  double Sg[MAXNG][MAXN1];      // l_i at Gauss-Lobatto(ng) points
+  double Sgs[2][MAXNG][MAXN1];  // l_i at (t + s)/2, t in GL(ng): coarse side of subface s
+  double wg[MAXNG];
+  double Ti[MAXN1][MAXN1];      // inverse of [l_i(Gauss(n1)_a)]_{a,i}, stored [i][a]
+  double wn[MAXN1];             // Gauss(n1) weights
+  double lumped[MAXN1];         // sum_a Sg[a][i] wg[a]
+};
+
+enum FaceKind
+{
+  FK_CONF = 0,    // same-level neighbour
+  FK_COARSE = 1,  // this cell is the coarse side of two subfaces
+  FK_FINE = 2,    // this cell is the fine side; neighbour coarse (subface index in bits 2)
+  FK_BDRY = 3
+};
+
+struct Field
+{
+  const double *ptr;  // stored values, or nullptr when the field is uniform
+  double cval;        // the uniform value
+};
+
+struct OpArgs
+{
+  int n_cells;
+  int n_batches;
+  const double *hxy;    // [batch][2][lane]   hx, hy
+  const int *nbr;       // [batch][4][2][lane] neighbour cell per face / subface
+  const int *fid;       // [batch][4][2][lane] inner face id or boundary face id
+  const int *finfo;     // [batch][4][lane]    kind | subface << 2
+  Field cell0[5];       // [batch][nq^2][lane] z_b, manning, tau_x, tau_y, coriolis
+  Field cell1;          // [batch][ng^2][lane] z_b at GL points
+  Field facez;          // [face][ng][2]
+  Field bdryz;          // [bface][nq]
+  const double *bvals;  // [bface][nq][3]
+  const int *bkind;     // [bface]
+  double g, cu4, nu;
+  int has_friction, has_wind, has_coriolis;
+  int max_it;
+  int xcd_swizzle;
+};
+
+struct StageArgs
+{
+  const double *src_h, *src_q, *src_v;
+  double *dst;               // L / M^-1 L output (ki[stage+1] or apply / rhs output)
+  double *scratch;           // optional ki[0] output
+  const double *kprev[MAXKI];// ki[1..stage] for the RK combination
+  double factor[MAXKI];
+  int stage;                 // number of previous k vectors = stage
+  const double *wsol;        // solution (w_h or w_q)
+  double *rk_dst;            // r or w
+  unsigned long long *sweeps;
+};
+
+__host__ __device__ constexpr int nq_of(int p) { return (3 * p) / 2 + 1; }
+
+template <int N1>
+__host__ __device__ constexpr int face_dof(int f, int k)
+{
+  return f == 0 ? N1 * k : f == 1 ? (N1 - 1) + N1 * k : f == 2 ? k : k + N1 * (N1 - 1);
+}
+
+// block -> batch so that each XCD (blocks are dealt round-robin over 8 XCDs) works on a
+// contiguous range of batches, keeping neighbour traces in its own L2
+__device__ __forceinline__ int logical_batch(int b, int nb, int swz)
+{
+  if (!swz)
+    return b;
+  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+__device__ __forceinline__ size_t vix(int batch, int ncomp, int comp, int nd, int k, int lane)
+{
+  return (((size_t)batch * ncomp + comp) * nd + k) * CB + lane;
+}
+
+__device__ __forceinline__ double fld(const Field &f, size_t i)
+{
+  return f.ptr ? f.ptr[i] : f.cval;
+}
+
+}  // namespace oceano

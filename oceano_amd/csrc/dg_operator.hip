@@ -1,0 +1,867 @@
+// dg_operator.hip -- host side of liboceano_dg.so: operator set-up, static data, launches.
+//
+// Mirrors SpaceDiscretization::OceanoOperator (OceanDG.h:157-380): create = constructor +
+// reinit + initialize_vector (438-504); set_static_data = initialize_data_at_quadrature
+// (498-569) with the function evaluation done by the caller at the exported points;
+// perform_stage_hydro / apply as declared at OceanDG.h:178-194.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "dg_internal.hpp"
+#include "oceano_dg.h"
+#include "rules.hpp"
+
+namespace oceano {
+
+thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+DegreeTables make_tables(int p)
+{
+  DegreeTables t;
+  std::memset(&t, 0, sizeof(t));
+  const int n1 = p + 1, nq = nq_of(p), ng = p + 2;
+  const Rule1D nodes = gauss_lobatto(n1), rq = gauss(nq), rg = gauss_lobatto(ng), rn = gauss(n1);
+  for (int a = 0; a < nq; ++a)
+    {
+      t.wq[a] = rq.w[a];
+      for (int i = 0; i < n1; ++i)
+        {
+          t.Sq[a][i] = lagrange_value(nodes.x, i, rq.x[a]);
+          t.Dq[a][i] = lagrange_slope(nodes.x, i, rq.x[a]);
+        }
+    }
+  for (int a = 0; a < ng; ++a)
+    {
+      t.wg[a] = rg.w[a];
+      for (int i = 0; i < n1; ++i)
+        {
+          t.Sg[a][i] = lagrange_value(nodes.x, i, rg.x[a]);
+          for (int s = 0; s < 2; ++s)
+            t.Sgs[s][a][i] = lagrange_value(nodes.x, i, 0.5 * (rg.x[a] + s));
+        }
+    }
+  std::vector<double> S(n1 * n1);
+  for (int a = 0; a < n1; ++a)
+    {
+      t.wn[a] = rn.w[a];
+      for (int i = 0; i < n1; ++i)
+        S[a * n1 + i] = lagrange_value(nodes.x, i, rn.x[a]);
+    }
+  const std::vector<double> Si = small_inverse(S, n1);  // Si[i][a]
+  for (int i = 0; i < n1; ++i)
+    for (int a = 0; a < n1; ++a)
+      t.Ti[i][a] = Si[i * n1 + a];
+  for (int i = 0; i < n1; ++i)
+    {
+      double s = 0;
+      for (int a = 0; a < ng; ++a)
+        s += t.Sg[a][i] * t.wg[a];
+      t.lumped[i] = s;
+    }
+  return t;
+}
+
+bool g_tables_ready[64] = {};
+
+int ensure_tables()
+{
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64)
+    {
+      set_error("device index out of range");
+      return OC_ERR_HIP;
+    }
+  if (g_tables_ready[dev])
+    return OC_OK;
+  DegreeTables t[4];
+  for (int p = 1; p <= 4; ++p)
+    t[p - 1] = make_tables(p);
+  int rc;
+  if ((rc = upload_tables_d1(t)) || (rc = upload_tables_d2(t)) || (rc = upload_tables_d3(t)) ||
+      (rc = upload_tables_d4(t)))
+    return rc;
+  g_tables_ready[dev] = true;
+  return OC_OK;
+}
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+  *p = nullptr;
+  if (n == 0)
+    return OC_OK;
+  HIP_TRY(hipMalloc((void **)p, n * sizeof(T)));
+  return OC_OK;
+}
+
+template <typename T>
+int upload(T **p, const std::vector<T> &h)
+{
+  if (*p)
+    {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  if (h.empty())
+    return OC_OK;
+  int rc = dalloc(p, h.size());
+  if (rc)
+    return rc;
+  HIP_TRY(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return OC_OK;
+}
+
+__global__ void k_pack(int n_cells, int ncomp, int nd, const double *__restrict__ in,
+                       double *__restrict__ out, int unpack)
+{
+  const size_t n_batches = ((size_t)n_cells + CB - 1) / CB;
+  const size_t total = n_batches * CB * ncomp * nd;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x)
+    {
+      // t enumerates the operator order: ((batch*ncomp + comp)*nd + k)*CB + lane
+      const int lane = t % CB;
+      const size_t r = t / CB;
+      const int k = r % nd;
+      const size_t r2 = r / nd;
+      const int comp = r2 % ncomp;
+      const size_t batch = r2 / ncomp;
+      const size_t cell = batch * CB + lane;
+      if (cell >= (size_t)n_cells)
+        {
+          if (!unpack)
+            out[t] = 0.0;  // padding lanes
+          continue;
+        }
+      const size_t cm = (cell * ncomp + comp) * nd + k;
+      if (unpack)
+        out[cm] = in[t];
+      else
+        out[t] = in[cm];
+    }
+}
+
+__global__ void k_zero_count(unsigned long long *p) { *p = 0ull; }
+
+__global__ void k_count_nonfinite(const double *__restrict__ x, size_t n,
+                                  unsigned long long *count)
+{
+  unsigned long long local = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    local += !isfinite(x[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    local += __shfl_xor(local, off);
+  if ((threadIdx.x & 63) == 0 && local)
+    atomicAdd(count, local);
+}
+
+}  // namespace
+}  // namespace oceano
+
+using namespace oceano;
+
+namespace {
+
+void free_all(oc_dg *op)
+{
+  auto f = [](void *p) {
+    if (p)
+      (void)hipFree(p);
+  };
+  f(op->d_hxy);
+  f(op->d_nbr);
+  f(op->d_fid);
+  f(op->d_finfo);
+  f(op->d_bkind);
+  for (auto p : op->d_cell0)
+    f(p);
+  f(op->d_cell1);
+  f(op->d_face);
+  f(op->d_bz);
+  f(op->d_bvals);
+  f(op->d_sweeps);
+  f(op->d_count);
+}
+
+// face point on local face f of cell geometry g at face parameter t in [0,1]
+void face_point(const double *g, int f, double t, double &x, double &y)
+{
+  switch (f)
+    {
+      case 0:
+        x = g[0];
+        y = g[1] + t * g[3];
+        break;
+      case 1:
+        x = g[0] + g[2];
+        y = g[1] + t * g[3];
+        break;
+      case 2:
+        x = g[0] + t * g[2];
+        y = g[1];
+        break;
+      default:
+        x = g[0] + t * g[2];
+        y = g[1] + g[3];
+        break;
+    }
+}
+
+bool uniform(const std::vector<double> &v, double &val)
+{
+  if (v.empty())
+    {
+      val = 0.0;
+      return true;
+    }
+  for (size_t i = 1; i < v.size(); ++i)
+    if (std::memcmp(&v[i], &v[0], sizeof(double)) != 0)
+      return false;
+  val = v[0];
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *oc_last_error(void) { return oceano::g_last_error.c_str(); }
+
+int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
+{
+  if (!mesh || !prm || !out || mesh->n_cells <= 0 || !mesh->cell_geom ||
+      (mesh->n_inner > 0 && !mesh->inner) || (mesh->n_boundary > 0 && !mesh->boundary))
+    {
+      set_error("oc_dg_create: null or empty argument");
+      return OC_ERR_ARGUMENT;
+    }
+  if (prm->degree < 1 || prm->degree > 4)
+    {
+      set_error("oc_dg_create: degree must be in 1..4");
+      return OC_ERR_ARGUMENT;
+    }
+  if (prm->precision != OC_PRECISION_FP64)
+    {
+      set_error("oc_dg_create: requested precision is not built");
+      return OC_ERR_UNSUPPORTED;
+    }
+  if (prm->max_iteration_height < 1)
+    {
+      set_error("oc_dg_create: max_iteration_height must be >= 1");
+      return OC_ERR_ARGUMENT;
+    }
+  int rc = OC_OK;
+  oc_dg *op = new oc_dg;
+  op->P = prm->degree;
+  op->n1 = op->P + 1;
+  op->nd = op->n1 * op->n1;
+  op->nq = nq_of(op->P);
+  op->ng = op->P + 2;
+  op->N = mesh->n_cells;
+  op->NB = (op->N + CB - 1) / CB;
+  op->precision = prm->precision;
+  op->g = prm->g;
+  const double cu = prm->threshold_wetdry;
+  op->cu4 = cu * cu * cu * cu;
+  op->nu = prm->viscosity;
+  op->max_it = prm->max_iteration_height;
+  op->geom.assign(mesh->cell_geom, mesh->cell_geom + 4 * (size_t)op->N);
+  for (int i = 0; i < prm->n_boundary_kinds; ++i)
+    {
+      const int kind = prm->boundary_kinds[i];
+      if (kind < OC_BC_WALL || kind > OC_BC_ABSORBING_OUTFLOW)
+        {
+          delete op;
+          set_error("oc_dg_create: unknown boundary kind");
+          return OC_ERR_ARGUMENT;
+        }
+      if (op->bc_of_id.count(prm->boundary_ids[i]))
+        {
+          delete op;
+          set_error("You already set the boundary with id " +
+                    std::to_string(prm->boundary_ids[i]) + " to another type of boundary");
+          return OC_ERR_BOUNDARY;
+        }
+      op->bc_of_id[prm->boundary_ids[i]] = kind;
+    }
+  const int N = op->N, NB = op->NB;
+  // topology -> per-cell face tables
+  std::vector<int> finfo((size_t)NB * 4 * CB, -1), nbr((size_t)NB * 4 * 2 * CB, 0),
+    fid((size_t)NB * 4 * 2 * CB, 0);
+  std::vector<int> slots((size_t)N * 4, 0);
+  auto at4 = [&](int c, int f) { return ((size_t)(c / CB) * 4 + f) * CB + c % CB; };
+  auto at8 = [&](int c, int f, int s) {
+    return (((size_t)(c / CB) * 4 + f) * 2 + s) * CB + c % CB;
+  };
+  auto fail = [&](const std::string &m, int code) {
+    delete op;
+    set_error(m);
+    return code;
+  };
+  auto set_kind = [&](int c, int f, int kind) -> bool {
+    int &k = finfo[at4(c, f)];
+    const int base = k < 0 ? -1 : (k & 3);
+    if (base >= 0 && base != (kind & 3))
+      return false;
+    k = kind;
+    return true;
+  };
+  op->inner_in.assign(mesh->inner, mesh->inner + 6 * (size_t)mesh->n_inner);
+  op->inner.resize(6 * (size_t)mesh->n_inner);
+  op->swapped.assign(mesh->n_inner, 0);
+  for (int i = 0; i < mesh->n_inner; ++i)
+    {
+      int cm = mesh->inner[6 * i], fm = mesh->inner[6 * i + 1], sm = mesh->inner[6 * i + 2];
+      int cp = mesh->inner[6 * i + 3], fp = mesh->inner[6 * i + 4], sp = mesh->inner[6 * i + 5];
+      if (cm < 0 || cm >= N || cp < 0 || cp >= N || fm < 0 || fm > 3 || fp < 0 || fp > 3 ||
+          sm < -1 || sm > 1 || sp < -1 || sp > 1 || (sm >= 0 && sp >= 0))
+        return fail("inner face " + std::to_string(i) + " is malformed", OC_ERR_MESH);
+      if ((fm & 1) == 0)
+        {
+          std::swap(cm, cp);
+          std::swap(fm, fp);
+          std::swap(sm, sp);
+          op->swapped[i] = 1;
+        }
+      if (fp != fm - 1)
+        return fail("inner face " + std::to_string(i) + " joins non-opposite faces", OC_ERR_MESH);
+      int *row = &op->inner[6 * i];
+      row[0] = cm;
+      row[1] = fm;
+      row[2] = sm;
+      row[3] = cp;
+      row[4] = fp;
+      row[5] = sp;
+      bool ok = true;
+      if (sm < 0 && sp < 0)
+        {
+          ok = set_kind(cm, fm, FK_CONF) && set_kind(cp, fp, FK_CONF);
+          nbr[at8(cm, fm, 0)] = cp;
+          fid[at8(cm, fm, 0)] = i;
+          nbr[at8(cp, fp, 0)] = cm;
+          fid[at8(cp, fp, 0)] = i;
+          slots[4 * cm + fm] += 1;
+          slots[4 * cp + fp] += 1;
+        }
+      else if (sm >= 0)
+        {
+          ok = set_kind(cm, fm, FK_COARSE) && set_kind(cp, fp, FK_FINE | (sm << 2));
+          nbr[at8(cm, fm, sm)] = cp;
+          fid[at8(cm, fm, sm)] = i;
+          nbr[at8(cp, fp, 0)] = cm;
+          fid[at8(cp, fp, 0)] = i;
+          slots[4 * cm + fm] += 1 << (4 + sm);
+          slots[4 * cp + fp] += 1;
+        }
+      else
+        {
+          ok = set_kind(cm, fm, FK_FINE | (sp << 2)) && set_kind(cp, fp, FK_COARSE);
+          nbr[at8(cm, fm, 0)] = cp;
+          fid[at8(cm, fm, 0)] = i;
+          nbr[at8(cp, fp, sp)] = cm;
+          fid[at8(cp, fp, sp)] = i;
+          slots[4 * cm + fm] += 1;
+          slots[4 * cp + fp] += 1 << (4 + sp);
+        }
+      if (!ok)
+        return fail("cell face used by faces of different kinds (inner face " +
+                      std::to_string(i) + ")",
+                    OC_ERR_MESH);
+    }
+  op->bdry.assign(mesh->boundary, mesh->boundary + 3 * (size_t)mesh->n_boundary);
+  std::vector<int> bkind(mesh->n_boundary);
+  for (int b = 0; b < mesh->n_boundary; ++b)
+    {
+      const int c = mesh->boundary[3 * b], f = mesh->boundary[3 * b + 1],
+                id = mesh->boundary[3 * b + 2];
+      if (c < 0 || c >= N || f < 0 || f > 3)
+        return fail("boundary face " + std::to_string(b) + " is malformed", OC_ERR_MESH);
+      auto it = op->bc_of_id.find(id);
+      if (it == op->bc_of_id.end())
+        return fail("Unknown boundary id " + std::to_string(id) +
+                      ", did you set a boundary condition for this part of the domain boundary?",
+                    OC_ERR_BOUNDARY);
+      bkind[b] = it->second;
+      if (!set_kind(c, f, FK_BDRY))
+        return fail("boundary face " + std::to_string(b) + " overlaps an inner face", OC_ERR_MESH);
+      fid[at8(c, f, 0)] = b;
+      slots[4 * c + f] += 1;
+    }
+  for (int c = 0; c < N; ++c)
+    for (int f = 0; f < 4; ++f)
+      {
+        const int k = finfo[at4(c, f)];
+        const int s = slots[4 * c + f];
+        const bool ok = (k >= 0) && (((k & 3) == FK_COARSE) ? (s == 0x30) : (s == 1));
+        if (!ok)
+          return fail("face " + std::to_string(f) + " of cell " + std::to_string(c) +
+                        " is not covered exactly once",
+                      OC_ERR_MESH);
+      }
+  for (auto &k : finfo)
+    if (k < 0)
+      k = FK_CONF;  // padding lanes
+  std::vector<double> hxy((size_t)NB * 2 * CB, 1.0);
+  for (int c = 0; c < N; ++c)
+    {
+      hxy[((size_t)(c / CB) * 2 + 0) * CB + c % CB] = op->geom[4 * c + 2];
+      hxy[((size_t)(c / CB) * 2 + 1) * CB + c % CB] = op->geom[4 * c + 3];
+    }
+  if ((rc = ensure_tables()))
+    {
+      delete op;
+      return rc;
+    }
+  if ((rc = upload(&op->d_hxy, hxy)) || (rc = upload(&op->d_nbr, nbr)) ||
+      (rc = upload(&op->d_fid, fid)) || (rc = upload(&op->d_finfo, finfo)) ||
+      (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps, 1)) ||
+      (rc = dalloc(&op->d_count, 1)))
+    {
+      free_all(op);
+      delete op;
+      return rc;
+    }
+  if (mesh->n_boundary > 0)
+    {
+      std::vector<double> bv((size_t)mesh->n_boundary * op->nq * 3, 0.0);
+      if ((rc = upload(&op->d_bvals, bv)))
+        {
+          free_all(op);
+          delete op;
+          return rc;
+        }
+    }
+  *out = op;
+  return OC_OK;
+}
+
+int oc_dg_destroy(oc_dg *op)
+{
+  if (!op)
+    return OC_ERR_ARGUMENT;
+  free_all(op);
+  delete op;
+  return OC_OK;
+}
+
+int oc_dg_sizes(const oc_dg *op, int64_t sizes[7])
+{
+  if (!op || !sizes)
+    return OC_ERR_ARGUMENT;
+  sizes[0] = op->N;
+  sizes[1] = (int64_t)op->NB * CB;
+  sizes[2] = op->nd;
+  sizes[3] = op->nq;
+  sizes[4] = op->ng;
+  sizes[5] = (int64_t)op->inner.size() / 6;
+  sizes[6] = (int64_t)op->bdry.size() / 3;
+  return OC_OK;
+}
+
+int oc_dg_vector_length(const oc_dg *op, int32_t n_components, int64_t *length)
+{
+  if (!op || !length || n_components < 1)
+    return OC_ERR_ARGUMENT;
+  *length = (int64_t)op->NB * CB * n_components * op->nd;
+  return OC_OK;
+}
+
+int oc_dg_n_points(const oc_dg *op, int32_t store, int64_t *n)
+{
+  if (!op || !n)
+    return OC_ERR_ARGUMENT;
+  const int64_t ni = (int64_t)op->inner.size() / 6, nb = (int64_t)op->bdry.size() / 3;
+  switch (store)
+    {
+      case OC_STORE_CELL_GAUSS:
+        *n = (int64_t)op->N * op->nq * op->nq;
+        return OC_OK;
+      case OC_STORE_CELL_LOBATTO:
+        *n = (int64_t)op->N * op->ng * op->ng;
+        return OC_OK;
+      case OC_STORE_CELL_COLLOC:
+        *n = (int64_t)op->N * op->n1 * op->n1;
+        return OC_OK;
+      case OC_STORE_FACE:
+        *n = ni * op->ng * 2;
+        return OC_OK;
+      case OC_STORE_BOUNDARY:
+        *n = nb * op->nq;
+        return OC_OK;
+      default:
+        set_error("unknown store");
+        return OC_ERR_ARGUMENT;
+    }
+}
+
+int oc_dg_quadrature_points(const oc_dg *op, int32_t store, double *xy)
+{
+  if (!op || !xy)
+    return OC_ERR_ARGUMENT;
+  const int N = op->N;
+  auto cell_rule = [&](const Rule1D &r) {
+    const int m = (int)r.x.size();
+    size_t o = 0;
+    for (int c = 0; c < N; ++c)
+      {
+        const double *g = &op->geom[4 * c];
+        for (int b = 0; b < m; ++b)
+          for (int a = 0; a < m; ++a)
+            {
+              xy[o++] = g[0] + r.x[a] * g[2];
+              xy[o++] = g[1] + r.x[b] * g[3];
+            }
+      }
+  };
+  switch (store)
+    {
+      case OC_STORE_CELL_GAUSS:
+        cell_rule(gauss(op->nq));
+        return OC_OK;
+      case OC_STORE_CELL_LOBATTO:
+        cell_rule(gauss_lobatto(op->ng));
+        return OC_OK;
+      case OC_STORE_CELL_COLLOC:
+        cell_rule(gauss(op->n1));
+        return OC_OK;
+      case OC_STORE_FACE:
+        {
+          const Rule1D r = gauss_lobatto(op->ng);
+          const size_t ni = op->inner_in.size() / 6;
+          size_t o = 0;
+          for (size_t i = 0; i < ni; ++i)
+            {
+              const int32_t *F = &op->inner_in[6 * i];
+              const double nx = F[1] == 0 ? -1.0 : F[1] == 1 ? 1.0 : 0.0;
+              const double ny = F[1] == 2 ? -1.0 : F[1] == 3 ? 1.0 : 0.0;
+              for (int a = 0; a < op->ng; ++a)
+                {
+                  double x, y;
+                  const double tm = F[2] >= 0 ? 0.5 * (r.x[a] + F[2]) : r.x[a];
+                  face_point(&op->geom[4 * F[0]], F[1], tm, x, y);
+                  xy[o++] = x - 1e-12 * nx;
+                  xy[o++] = y - 1e-12 * ny;
+                  const double tp = F[5] >= 0 ? 0.5 * (r.x[a] + F[5]) : r.x[a];
+                  face_point(&op->geom[4 * F[3]], F[4], tp, x, y);
+                  xy[o++] = x + 1e-12 * nx;
+                  xy[o++] = y + 1e-12 * ny;
+                }
+            }
+          return OC_OK;
+        }
+      case OC_STORE_BOUNDARY:
+        {
+          const Rule1D r = gauss(op->nq);
+          size_t o = 0;
+          for (size_t b = 0; b < op->bdry.size() / 3; ++b)
+            for (int a = 0; a < op->nq; ++a)
+              {
+                double x, y;
+                face_point(&op->geom[4 * op->bdry[3 * b]], op->bdry[3 * b + 1], r.x[a], x, y);
+                xy[o++] = x;
+                xy[o++] = y;
+              }
+          return OC_OK;
+        }
+      default:
+        set_error("unknown store");
+        return OC_ERR_ARGUMENT;
+    }
+}
+
+int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
+{
+  if (!op || !values)
+    return OC_ERR_ARGUMENT;
+  const int N = op->N, NB = op->NB;
+  int rc = OC_OK;
+  auto interleave = [&](const std::vector<double> &cm, int npts, double **dptr, double &cval) {
+    double u;
+    if (uniform(cm, u))
+      {
+        if (*dptr)
+          (void)hipFree(*dptr);
+        *dptr = nullptr;
+        cval = u;
+        return OC_OK;
+      }
+    std::vector<double> il((size_t)NB * npts * CB, 0.0);
+    for (int c = 0; c < N; ++c)
+      for (int q = 0; q < npts; ++q)
+        il[((size_t)(c / CB) * npts + q) * CB + c % CB] = cm[(size_t)c * npts + q];
+    cval = 0.0;
+    return upload(dptr, il);
+  };
+  switch (store)
+    {
+      case OC_STORE_CELL_GAUSS:
+        {
+          const int np = op->nq * op->nq;
+          for (int f = 0; f < 5 && !rc; ++f)
+            {
+              std::vector<double> cm((size_t)N * np);
+              for (size_t i = 0; i < cm.size(); ++i)
+                cm[i] = values[i * 5 + f];
+              rc = interleave(cm, np, &op->d_cell0[f], op->c_cell0[f]);
+            }
+          return rc;
+        }
+      case OC_STORE_CELL_LOBATTO:
+        {
+          const int np = op->ng * op->ng;
+          std::vector<double> cm(values, values + (size_t)N * np);
+          return interleave(cm, np, &op->d_cell1, op->c_cell1);
+        }
+      case OC_STORE_CELL_COLLOC:
+        op->cell2_host.assign(values, values + (size_t)N * op->nd * 2);
+        return OC_OK;
+      case OC_STORE_FACE:
+        {
+          const size_t ni = op->inner.size() / 6;
+          std::vector<double> v(values, values + ni * op->ng * 2);
+          for (size_t i = 0; i < ni; ++i)
+            if (op->swapped[i])
+              for (int a = 0; a < op->ng; ++a)
+                std::swap(v[(i * op->ng + a) * 2], v[(i * op->ng + a) * 2 + 1]);
+          double u;
+          if (uniform(v, u))
+            {
+              if (op->d_face)
+                (void)hipFree(op->d_face);
+              op->d_face = nullptr;
+              op->c_face = u;
+              return OC_OK;
+            }
+          op->c_face = 0.0;
+          return upload(&op->d_face, v);
+        }
+      case OC_STORE_BOUNDARY:
+        {
+          std::vector<double> v(values, values + (op->bdry.size() / 3) * op->nq);
+          double u;
+          if (uniform(v, u))
+            {
+              if (op->d_bz)
+                (void)hipFree(op->d_bz);
+              op->d_bz = nullptr;
+              op->c_bz = u;
+              return OC_OK;
+            }
+          op->c_bz = 0.0;
+          return upload(&op->d_bz, v);
+        }
+      default:
+        set_error("unknown store");
+        return OC_ERR_ARGUMENT;
+    }
+}
+
+int oc_dg_set_boundary_values(oc_dg *op, const double *values)
+{
+  if (!op || !values)
+    return OC_ERR_ARGUMENT;
+  const size_t n = (op->bdry.size() / 3) * op->nq * 3;
+  if (n == 0)
+    return OC_OK;
+  HIP_TRY(hipMemcpy(op->d_bvals, values, n * sizeof(double), hipMemcpyHostToDevice));
+  return OC_OK;
+}
+
+int oc_dg_pack(oc_dg *op, int32_t n_components, const double *cell_major, double *op_order,
+               void *stream)
+{
+  if (!op || n_components < 1 || !cell_major || !op_order)
+    return OC_ERR_ARGUMENT;
+  const size_t total = (size_t)op->NB * CB * n_components * op->nd;
+  hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, op->N,
+                     n_components, op->nd, cell_major, op_order, 0);
+  HIP_TRY(hipGetLastError());
+  return OC_OK;
+}
+
+int oc_dg_unpack(oc_dg *op, int32_t n_components, const double *op_order, double *cell_major,
+                 void *stream)
+{
+  if (!op || n_components < 1 || !cell_major || !op_order)
+    return OC_ERR_ARGUMENT;
+  const size_t total = (size_t)op->NB * CB * n_components * op->nd;
+  hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, op->N,
+                     n_components, op->nd, op_order, cell_major, 1);
+  HIP_TRY(hipGetLastError());
+  return OC_OK;
+}
+
+int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const double *factor,
+                              uint32_t n_factor, const double *ri_h, const double *ri_q,
+                              const double *ri_vel, double *const *ki_h, double *const *ki_q,
+                              uint32_t n_ki, double *w_h, double *w_q, double *r_h,
+                              double *r_q, int32_t write_scratch, void *stream)
+{
+  (void)time;
+  if (!op || !factor || !ri_h || !ri_q || !ki_h || !ki_q || !w_h || !w_q)
+    {
+      set_error("oc_dg_perform_stage_hydro: null argument");
+      return OC_ERR_ARGUMENT;
+    }
+  if (n_ki < 2 || n_ki > MAXKI || stage + 2 > n_ki || n_factor != stage + 1)
+    {
+      set_error("oc_dg_perform_stage_hydro: stage / n_ki / n_factor inconsistent");
+      return OC_ERR_ARGUMENT;
+    }
+  const bool last = stage == n_ki - 2;
+  if (!last && (!r_h || !r_q))
+    {
+      set_error("oc_dg_perform_stage_hydro: next_ri vectors required before the last stage");
+      return OC_ERR_ARGUMENT;
+    }
+  if (op->nu != 0.0 && !ri_vel)
+    {
+      set_error("oc_dg_perform_stage_hydro: velocity vector required when viscosity != 0");
+      return OC_ERR_ARGUMENT;
+    }
+  for (uint32_t j = 0; j <= stage + 1; ++j)
+    if (!ki_h[j] || !ki_q[j])
+      {
+        if (j == 0 && !write_scratch)
+          continue;
+        set_error("oc_dg_perform_stage_hydro: null ki vector");
+        return OC_ERR_ARGUMENT;
+      }
+  const OpArgs O = op->args();
+  hipStream_t s = (hipStream_t)stream;
+  switch (op->P)
+    {
+      case 1:
+        return stage_d1(op, O, stage, factor, ri_h, ri_q, ri_vel, ki_h, ki_q, n_ki, w_h,
+                               w_q, r_h, r_q, write_scratch, s);
+      case 2:
+        return stage_d2(op, O, stage, factor, ri_h, ri_q, ri_vel, ki_h, ki_q, n_ki, w_h,
+                               w_q, r_h, r_q, write_scratch, s);
+      case 3:
+        return stage_d3(op, O, stage, factor, ri_h, ri_q, ri_vel, ki_h, ki_q, n_ki, w_h,
+                               w_q, r_h, r_q, write_scratch, s);
+      default:
+        return stage_d4(op, O, stage, factor, ri_h, ri_q, ri_vel, ki_h, ki_q, n_ki, w_h,
+                               w_q, r_h, r_q, write_scratch, s);
+    }
+}
+
+static int rhs_or_apply(oc_dg *op, bool invert, const double *src_h, const double *src_q,
+                        const double *src_vel, double *dst_h, double *dst_q, void *stream)
+{
+  if (!op || !src_h || !src_q || !dst_h || !dst_q)
+    return OC_ERR_ARGUMENT;
+  if (op->nu != 0.0 && !src_vel)
+    {
+      set_error("velocity vector required when viscosity != 0");
+      return OC_ERR_ARGUMENT;
+    }
+  const OpArgs O = op->args();
+  hipStream_t s = (hipStream_t)stream;
+  switch (op->P)
+    {
+      case 1:
+        return rhs_apply_d1(op, O, invert, src_h, src_q, src_vel, dst_h, dst_q, s);
+      case 2:
+        return rhs_apply_d2(op, O, invert, src_h, src_q, src_vel, dst_h, dst_q, s);
+      case 3:
+        return rhs_apply_d3(op, O, invert, src_h, src_q, src_vel, dst_h, dst_q, s);
+      default:
+        return rhs_apply_d4(op, O, invert, src_h, src_q, src_vel, dst_h, dst_q, s);
+    }
+}
+
+int oc_dg_apply(oc_dg *op, double time, const double *src_h, const double *src_q,
+                const double *src_vel, double *dst_h, double *dst_q, void *stream)
+{
+  (void)time;
+  return rhs_or_apply(op, true, src_h, src_q, src_vel, dst_h, dst_q, stream);
+}
+
+int oc_dg_rhs(oc_dg *op, const double *src_h, const double *src_q, const double *src_vel,
+              double *dst_h, double *dst_q, void *stream)
+{
+  return rhs_or_apply(op, false, src_h, src_q, src_vel, dst_h, dst_q, stream);
+}
+
+int oc_dg_project_collocation(const oc_dg *op, int32_t n_comp, const double *values,
+                              double *coeffs)
+{
+  if (!op || n_comp < 1 || !values || !coeffs)
+    return OC_ERR_ARGUMENT;
+  const int n1 = op->n1, nd = op->nd;
+  const Rule1D nodes = gauss_lobatto(n1), rn = gauss(n1);
+  std::vector<double> S(nd);
+  for (int a = 0; a < n1; ++a)
+    for (int i = 0; i < n1; ++i)
+      S[a * n1 + i] = lagrange_value(nodes.x, i, rn.x[a]);
+  const std::vector<double> Ti = small_inverse(S, n1);  // Ti[i][a]
+  std::vector<double> T(nd);
+  for (int c = 0; c < op->N; ++c)
+    for (int d = 0; d < n_comp; ++d)
+      {
+        const double *u = values + (size_t)c * nd * n_comp;
+        double *U = coeffs + ((size_t)c * n_comp + d) * nd;
+        for (int b = 0; b < n1; ++b)
+          for (int i = 0; i < n1; ++i)
+            {
+              double s = 0;
+              for (int a = 0; a < n1; ++a)
+                s += Ti[i * n1 + a] * u[(a + n1 * b) * n_comp + d];
+              T[i + n1 * b] = s;
+            }
+        for (int j = 0; j < n1; ++j)
+          for (int i = 0; i < n1; ++i)
+            {
+              double s = 0;
+              for (int b = 0; b < n1; ++b)
+                s += Ti[j * n1 + b] * T[i + n1 * b];
+              U[i + n1 * j] = s;
+            }
+      }
+  return OC_OK;
+}
+
+int oc_dg_count_nonfinite(oc_dg *op, const double *x, int64_t n, int64_t *count, void *stream)
+{
+  if (!op || !x || n < 0 || !count)
+    return OC_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, op->d_count);
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  if (n > 0)
+    hipLaunchKernelGGL(k_count_nonfinite, dim3(std::max(blocks, 1)), dim3(256), 0, s, x,
+                       (size_t)n, op->d_count);
+  unsigned long long h = 0;
+  HIP_TRY(hipMemcpyAsync(&h, op->d_count, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *count = (int64_t)h;
+  return OC_OK;
+}
+
+int oc_dg_last_sweeps(oc_dg *op, int64_t *sweeps, void *stream)
+{
+  if (!op || !sweeps)
+    return OC_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned long long h = 0;
+  HIP_TRY(hipMemcpyAsync(&h, op->d_sweeps, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *sweeps = (int64_t)h;
+  return OC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,34 @@
+// dg_stage_p4.hip -- kernels and launch sequences for degree 4 (one TU per degree).
+#include "dg_internal.hpp"
+
+namespace oceano {
+static __constant__ DegreeTables c_tab[4];
+}
+
+#include "dg_launch.hpp"
+
+namespace oceano {
+
+int upload_tables_d4(const DegreeTables *t)
+{
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), t, sizeof(DegreeTables) * 4));
+  return OC_OK;
+}
+
+int stage_d4(oc_dg *op, const OpArgs &O, uint32_t stage, const double *factor,
+             const double *ri_h, const double *ri_q, const double *ri_v, double *const *ki_h,
+             double *const *ki_q, uint32_t n_ki, double *w_h, double *w_q, double *r_h,
+             double *r_q, int write_scratch, hipStream_t s)
+{
+  return launch_stage<4>(op, O, stage, factor, ri_h, ri_q, ri_v, ki_h, ki_q, n_ki, w_h, w_q,
+                         r_h, r_q, write_scratch, s);
+}
+
+int rhs_apply_d4(oc_dg *op, const OpArgs &O, bool invert, const double *src_h,
+                 const double *src_q, const double *src_v, double *dst_h, double *dst_q,
+                 hipStream_t s)
+{
+  return launch_rhs_apply<4>(op, O, invert, src_h, src_q, src_v, dst_h, dst_q, s);
+}
+
+}  // namespace oceano

@@ -1,0 +1,205 @@
+"""OceanoOperator: the host-side mirror of SpaceDiscretization::OceanoOperator.
+
+Same member names, argument meaning and error behaviour as the reference class
+(OceanDG.h:157-380): construction + reinit set the operator up on a mesh,
+initialize_data_at_quadrature tabulates the problem data at the quadrature points
+(OceanDG.h:498-569), perform_stage_hydro runs one explicit Runge-Kutta stage
+(OceanDG.h:182-194, 1782-1912) and apply evaluates M^-1 L (OceanDG.h:178-180).
+Vectors are torch tensors on the GPU in the operator's storage order (see
+include/oceano_dg.h); every call goes through liboceano_dg.so, which raises on failure.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import check, dptr, lib
+
+BC_KINDS = {
+    "wall": _capi.OC_BC_WALL,
+    "supercritical_inflow": _capi.OC_BC_SUPERCRITICAL_INFLOW,
+    "supercritical_outflow": _capi.OC_BC_SUPERCRITICAL_OUTFLOW,
+    "height_inflow": _capi.OC_BC_HEIGHT_INFLOW,
+    "discharge_inflow": _capi.OC_BC_DISCHARGE_INFLOW,
+    "absorbing_outflow": _capi.OC_BC_ABSORBING_OUTFLOW,
+}
+
+STORES = {
+    "cell_0": _capi.OC_STORE_CELL_GAUSS,
+    "cell_1": _capi.OC_STORE_CELL_LOBATTO,
+    "cell_2": _capi.OC_STORE_CELL_COLLOC,
+    "face": _capi.OC_STORE_FACE,
+    "boundary": _capi.OC_STORE_BOUNDARY,
+}
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class OceanoOperator:
+    """GPU operator for one mesh and one polynomial degree (1..4)."""
+
+    def __init__(self, mesh, degree, g=9.81, threshold_wetdry=0.1, viscosity=0.0,
+                 max_iteration_height=15, precision="fp64", boundary_conditions=None,
+                 device="cuda"):
+        self.mesh = mesh
+        self.degree = degree
+        self.device = torch.device(device)
+        self.precision = precision
+        bcs = dict(boundary_conditions or {})
+        ids = np.array(list(bcs.keys()), np.int32)
+        kinds = np.array([BC_KINDS[k] if isinstance(k, str) else int(k) for k in bcs.values()],
+                         np.int32)
+        self._keep = (mesh.geom, mesh.inner, mesh.boundary, ids, kinds)
+        desc = _capi.MeshDesc(mesh.n_cells, dptr(mesh.geom), mesh.inner.shape[0],
+                              _capi.iptr(mesh.inner), mesh.boundary.shape[0],
+                              _capi.iptr(mesh.boundary))
+        prm = _capi.Params(degree, g, threshold_wetdry, viscosity, max_iteration_height,
+                           {"fp64": _capi.OC_PRECISION_FP64,
+                            "mixed": _capi.OC_PRECISION_MIXED}[precision],
+                           len(ids), _capi.iptr(ids), _capi.iptr(kinds))
+        self.g = g
+        self.viscosity = viscosity
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            check(lib().oc_dg_create(C.byref(desc), C.byref(prm), C.byref(h)), "oc_dg_create")
+        self._h = h
+        sizes = np.zeros(7, np.int64)
+        check(lib().oc_dg_sizes(self._h, sizes.ctypes.data_as(_capi.PI64)), "oc_dg_sizes")
+        (self.n_cells, self.n_cells_padded, self.dofs_per_cell, self.n_q_points_1d,
+         self.n_lobatto_1d, self.n_inner, self.n_boundary) = (int(v) for v in sizes)
+
+    # ---------------------------------------------------------------- set-up -------
+    def quadrature_points(self, store):
+        s = STORES[store]
+        n = np.zeros(1, np.int64)
+        check(lib().oc_dg_n_points(self._h, s, n.ctypes.data_as(_capi.PI64)), "oc_dg_n_points")
+        xy = np.zeros((int(n[0]), 2))
+        check(lib().oc_dg_quadrature_points(self._h, s, dptr(xy)), "oc_dg_quadrature_points")
+        return xy
+
+    def set_static_data(self, store, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().oc_dg_set_static_data(self._h, STORES[store], dptr(v)),
+              "oc_dg_set_static_data")
+
+    def initialize_data_at_quadrature(self, problem_data):
+        """problem_data(xy[n,2]) -> [n,5] = z_b, friction, tau_x, tau_y, coriolis."""
+        data = {}
+        data["cell_0"] = problem_data(self.quadrature_points("cell_0"))
+        data["cell_1"] = problem_data(self.quadrature_points("cell_1"))[:, 0]
+        data["cell_2"] = problem_data(self.quadrature_points("cell_2"))[:, :2]
+        data["face"] = problem_data(self.quadrature_points("face"))[:, 0]
+        data["boundary"] = problem_data(self.quadrature_points("boundary"))[:, 0]
+        for k, v in data.items():
+            self.set_static_data(k, v)
+        return data
+
+    def set_boundary_values(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().oc_dg_set_boundary_values(self._h, dptr(v)), "oc_dg_set_boundary_values")
+
+    # --------------------------------------------------------------- vectors -------
+    def vector_length(self, n_components):
+        n = np.zeros(1, np.int64)
+        check(lib().oc_dg_vector_length(self._h, n_components, n.ctypes.data_as(_capi.PI64)),
+              "oc_dg_vector_length")
+        return int(n[0])
+
+    def initialize_vector(self, variable):
+        """variable 0 = free surface (1 component), 1 = discharge / velocity (2)."""
+        nc = 1 if variable == 0 else 2
+        return torch.zeros(self.vector_length(nc), dtype=torch.float64, device=self.device)
+
+    def pack(self, cell_major, n_components):
+        src = torch.as_tensor(cell_major, dtype=torch.float64).to(self.device).contiguous()
+        dst = torch.empty(self.vector_length(n_components), dtype=torch.float64,
+                          device=self.device)
+        check(lib().oc_dg_pack(self._h, n_components, _vp(src), _vp(dst), _stream()),
+              "oc_dg_pack")
+        return dst
+
+    def unpack(self, op_vec, n_components):
+        dst = torch.empty(self.n_cells * n_components * self.dofs_per_cell, dtype=torch.float64,
+                          device=self.device)
+        check(lib().oc_dg_unpack(self._h, n_components, _vp(op_vec), _vp(dst), _stream()),
+              "oc_dg_unpack")
+        return dst
+
+    def project_collocation(self, values, n_components):
+        """Nodal coefficients (cell-major, host) from values at the cell_2 points."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        out = np.zeros(self.n_cells * n_components * self.dofs_per_cell)
+        check(lib().oc_dg_project_collocation(self._h, n_components, dptr(v), dptr(out)),
+              "oc_dg_project_collocation")
+        return out
+
+    def project_hydro(self, function, zb_cell2):
+        """OceanDG.h:2242-2352: function(xy) -> [n,3] (free surface, u, v); returns the
+        operator-order vectors (zeta, q) with q = (u, v) * depth(zeta, z_b)."""
+        xy = self.quadrature_points("cell_2")
+        w = function(xy)
+        zeta = w[:, 0]
+        h = np.maximum(zeta + zb_cell2, 1e-12)
+        qv = np.stack([w[:, 1] * h, w[:, 2] * h], axis=1)
+        zh = self.project_collocation(zeta, 1)
+        zq = self.project_collocation(qv, 2)
+        return self.pack(zh, 1), self.pack(zq, 2)
+
+    # ------------------------------------------------------------- the stage -------
+    def perform_stage_hydro(self, cur_stage, cur_time, factor_residual, current_ri,
+                            vec_ki_height, vec_ki_discharge, solution_height,
+                            solution_discharge, next_ri_height, next_ri_discharge,
+                            write_scratch=False):
+        """OceanDG.h:182-194.  current_ri = [zeta_r, q_r, tracer, velocity]."""
+        n_ki = len(vec_ki_height)
+        if len(vec_ki_discharge) != n_ki:
+            raise ValueError("vec_ki_height and vec_ki_discharge differ in length")
+        fac = np.ascontiguousarray(np.asarray(factor_residual, np.float64)[:cur_stage + 1])
+        kh = (C.c_void_p * n_ki)(*[k.data_ptr() for k in vec_ki_height])
+        kq = (C.c_void_p * n_ki)(*[k.data_ptr() for k in vec_ki_discharge])
+        vel = current_ri[-1] if len(current_ri) > 2 else None
+        check(lib().oc_dg_perform_stage_hydro(
+            self._h, cur_stage, cur_time, dptr(fac), cur_stage + 1, _vp(current_ri[0]),
+            _vp(current_ri[1]), _vp(vel), kh, kq, n_ki, _vp(solution_height),
+            _vp(solution_discharge), _vp(next_ri_height), _vp(next_ri_discharge),
+            int(write_scratch), _stream()), "oc_dg_perform_stage_hydro")
+
+    def apply(self, current_time, src, dst):
+        """dst = M^-1 L(src); src/dst = [zeta, q] (+ velocity in src when viscous)."""
+        vel = src[2] if len(src) > 2 else None
+        check(lib().oc_dg_apply(self._h, current_time, _vp(src[0]), _vp(src[1]), _vp(vel),
+                                _vp(dst[0]), _vp(dst[1]), _stream()), "oc_dg_apply")
+
+    def rhs(self, src, dst):
+        """dst = L(src) without inverse mass (the two MatrixFree::loop calls)."""
+        vel = src[2] if len(src) > 2 else None
+        check(lib().oc_dg_rhs(self._h, _vp(src[0]), _vp(src[1]), _vp(vel), _vp(dst[0]),
+                              _vp(dst[1]), _stream()), "oc_dg_rhs")
+
+    def count_nonfinite(self, vec):
+        n = np.zeros(1, np.int64)
+        check(lib().oc_dg_count_nonfinite(self._h, _vp(vec), vec.numel(),
+                                          n.ctypes.data_as(_capi.PI64), _stream()),
+              "oc_dg_count_nonfinite")
+        return int(n[0])
+
+    def last_sweeps(self):
+        n = np.zeros(1, np.int64)
+        check(lib().oc_dg_last_sweeps(self._h, n.ctypes.data_as(_capi.PI64), _stream()),
+              "oc_dg_last_sweeps")
+        return int(n[0])
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().oc_dg_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

@@ -1,0 +1,143 @@
+"""Test-case set-ups (ICBC layer): meshes, problem data, boundary data, initial states.
+
+Vortex (Icbc_ShallowWaterVortex.h): [0,2]x[0,1], 20x10 coarse cells (the fixture
+tests/shallowWaterVortex/shallowWaterVortex_20x10cells.msh is this structured grid with
+boundary ids 0 = walls y=0,1, 1 = x=0, 2 = x=2), refined globally, optionally with a seeded
+fraction refined once more (hanging nodes); periodic, or with the reference's
+supercritical inflow / outflow / wall conditions (Icbc_ShallowWaterVortex.h:262-271).
+Channel (Icbc_ChannelFlow.h BC kinds): discharge inflow at x=0, height condition at the
+outflow, walls at y=0,1, Manning friction, a bump plus seeded sub-cell bumps.
+"""
+import numpy as np
+
+from . import _capi
+from ._capi import check, dptr, lib
+from .mesh import Mesh
+
+
+def vortex_exact(xy, t, prm):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    out = np.zeros((xy.shape[0], 3))
+    p = np.ascontiguousarray(prm, dtype=np.float64)
+    check(lib().oc_icbc_vortex(dptr(p), t, xy.shape[0], dptr(xy), dptr(out)), "oc_icbc_vortex")
+    return out
+
+
+def channel_data(xy, prm, seed):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    out = np.zeros((xy.shape[0], 5))
+    p = np.ascontiguousarray(prm, dtype=np.float64)
+    check(lib().oc_icbc_channel(dptr(p), seed, xy.shape[0], dptr(xy), dptr(out)),
+          "oc_icbc_channel")
+    return out
+
+
+class Problem:
+    """mesh, operator parameters, problem data, boundary data and initial condition."""
+
+    def __init__(self, mesh, degree, bcs, problem_data, initial, boundary_values=None,
+                 exact=None, g=9.81, viscosity=0.0, name=""):
+        self.mesh = mesh
+        self.degree = degree
+        self.bcs = bcs
+        self.problem_data = problem_data    # xy -> [n,5]
+        self.initial = initial              # xy -> [n,3] free surface, u, v
+        self.boundary_values = boundary_values  # (op, t) -> [n_bdry, nq, 3]
+        self.exact = exact                  # (xy, t) -> [n,3] or None
+        self.g = g
+        self.viscosity = viscosity
+        self.name = name
+
+    def dofs(self):
+        return 3 * self.mesh.n_cells * (self.degree + 1) ** 2
+
+
+def vortex(degree=1, refine=3, refine_fraction=None, seed=20261202, periodic=True,
+           u_inf=(1.0, 0.0), amplitude=0.1, radius=0.35, center=(0.5, 0.5), h0=1.0,
+           g=9.81, nx=20, ny=10):
+    mesh = Mesh.rectangle(0.0, 2.0, 0.0, 1.0, nx, ny, periodic=(periodic, periodic),
+                          boundary_ids=(1, 2, 0, 0))
+    if refine:
+        mesh.refine_global(refine)
+    if refine_fraction:
+        mesh.refine_fraction(refine_fraction, seed)
+    prm = [h0, u_inf[0], u_inf[1], amplitude, radius, center[0], center[1], g,
+           2.0 if periodic else 0.0, 1.0 if periodic else 0.0]
+
+    def exact(xy, t):
+        return vortex_exact(xy, t, prm)
+
+    def problem_data(xy):
+        return np.zeros((xy.shape[0], 5))
+
+    bcs = {} if periodic else {0: "wall", 1: "supercritical_inflow", 2: "supercritical_outflow"}
+
+    def boundary_values(op, t):
+        xy = op.quadrature_points("boundary")
+        return exact(xy, t).reshape(op.n_boundary, op.n_q_points_1d, 3)
+
+    return Problem(mesh, degree, bcs, problem_data, lambda xy: exact(xy, 0.0),
+                   None if periodic else boundary_values, exact, g=g,
+                   name="vortex_p%d" % degree)
+
+
+def channel(degree=2, nx=250, ny=10, seed=7, manning=0.02, discharge=4.42, h_out=2.0,
+            g=9.81, n_bumps=-1):
+    mesh = Mesh.rectangle(0.0, 25.0, 0.0, 1.0, nx, ny, boundary_ids=(1, 2, 0, 0))
+    prm = [manning, n_bumps, 0.02, 0.05, 0.02, 0.05, 1.0, 24.0]
+
+    def problem_data(xy):
+        return channel_data(xy, prm, seed)
+
+    def initial(xy):
+        zb = problem_data(xy)[:, 0]
+        h = np.maximum(h_out + zb, 1e-12)
+        return np.stack([np.full(xy.shape[0], h_out), discharge / h, np.zeros(xy.shape[0])], 1)
+
+    def boundary_values(op, t):
+        v = np.zeros((op.n_boundary, op.n_q_points_1d, 3))
+        v[:, :, 0] = h_out
+        v[:, :, 1] = discharge
+        return v
+
+    bcs = {0: "wall", 1: "discharge_inflow", 2: "height_inflow"}
+    return Problem(mesh, degree, bcs, problem_data, initial, boundary_values, None, g=g,
+                   name="channel_p%d" % degree)
+
+
+def lake_at_rest(degree=2, nx=8, ny=4, level=1.0, seed=3):
+    """zeta = const, q = 0 over a bed with smooth and discontinuous parts
+    (Icbc_LakeAtRest.h:45-77): the well-balanced operator returns L = 0."""
+    mesh = Mesh.rectangle(0.0, 2.0, 0.0, 1.0, nx, ny, boundary_ids=(0, 0, 0, 0))
+    rng = np.random.default_rng(seed)
+    steps = rng.uniform(0.2, 0.8, size=4)
+
+    def problem_data(xy):
+        x, y = xy[:, 0], xy[:, 1]
+        zb = 0.5 + 0.2 * np.sin(3 * x) * np.cos(2 * y) + np.where(x > 0.73, 0.1, 0.0)
+        out = np.zeros((xy.shape[0], 5))
+        out[:, 0] = zb
+        out[:, 1] = 0.03
+        return out
+
+    def initial(xy):
+        return np.stack([np.full(xy.shape[0], level), np.zeros(xy.shape[0]),
+                         np.zeros(xy.shape[0])], 1)
+
+    return Problem(mesh, degree, {0: "wall"}, problem_data, initial, None, None,
+                   name="lake_at_rest")
+
+
+def setup_operator(problem, precision="fp64", device="cuda", max_iteration_height=15):
+    """Build the GPU operator, tabulate the static data and project the initial state."""
+    from .operator import OceanoOperator
+    op = OceanoOperator(problem.mesh, problem.degree, g=problem.g,
+                        viscosity=problem.viscosity, precision=precision,
+                        boundary_conditions=problem.bcs, device=device,
+                        max_iteration_height=max_iteration_height)
+    data = op.initialize_data_at_quadrature(problem.problem_data)
+    if problem.boundary_values is not None and op.n_boundary:
+        op.set_boundary_values(problem.boundary_values(op, 0.0))
+    zb2 = data["cell_2"][:, 0]
+    h, q = op.project_hydro(problem.initial, zb2)
+    return op, data, h, q

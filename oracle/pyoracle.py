@@ -1,0 +1,216 @@
+"""ctypes wrapper of liboceano_oracle.so, the CPU restatement of the reference stage.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker.  Vectors are numpy float64 arrays in cell-major
+order (cell, component, node) -- deal.II's FESystem(FE_DGQ(p), c) numbering.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboceano_oracle.so")
+
+BC = {"wall": 1, "supercritical_inflow": 2, "supercritical_outflow": 3, "height_inflow": 4,
+      "discharge_inflow": 5, "absorbing_outflow": 6}
+
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "ocr_last_error": (C.c_char_p, []),
+            "ocr_rule": (None, [C.c_int, C.c_int, PD, PD]),
+            "ocr_lagrange": (None, [C.c_int, PD, C.c_int, PD, PD, PD]),
+            "ocr_create": (C.c_void_p, [C.c_int, C.c_int, PD, C.c_int, PI, C.c_int, PI, PI, PD]),
+            "ocr_destroy": (None, [C.c_void_p]),
+            "ocr_set_static": (C.c_int, [C.c_void_p, C.c_int, PD]),
+            "ocr_set_lane_width": (None, [C.c_void_p, C.c_int]),
+            "ocr_set_max_iterations": (None, [C.c_void_p, C.c_int]),
+            "ocr_last_sweeps": (C.c_long, [C.c_void_p]),
+            "ocr_rhs": (C.c_int, [C.c_void_p, PD, PD, PD, PD, PD]),
+            "ocr_cell_mass_height": (C.c_int, [C.c_void_p, PD, PD]),
+            "ocr_inverse_mass_height": (C.c_int, [C.c_void_p, PD, PD]),
+            "ocr_inverse_mass_tensor": (C.c_int, [C.c_void_p, PD, PD, C.c_int]),
+            "ocr_perform_stage_hydro": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PD, PD, PD,
+                                                  C.POINTER(PD), C.POINTER(PD), PD, PD, PD, PD]),
+            "ocr_quadrature_points": (C.c_int, [C.c_void_p, C.c_int, PD]),
+            "ocr_project_collocation": (C.c_int, [C.c_void_p, C.c_int, PD, PD]),
+            "ocr_errors": (C.c_int, [C.c_void_p, PD, PD, PD, PD]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(PD)
+
+
+def rule(kind, n):
+    """kind 'gauss' | 'lobatto' on [0,1]"""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().ocr_rule(0 if kind == "gauss" else 1, n, _d(x), _d(w))
+    return x, w
+
+
+def lagrange(nodes, pts):
+    nodes = np.ascontiguousarray(nodes, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64)
+    v = np.zeros((len(pts), len(nodes)))
+    d = np.zeros((len(pts), len(nodes)))
+    lib().ocr_lagrange(len(nodes), _d(nodes), len(pts), _d(pts), _d(v), _d(d))
+    return v, d
+
+
+STORE = {"cell_0": 0, "cell_1": 1, "cell_2": 2, "face": 3, "boundary": 4, "bvals": 5}
+
+
+class Oracle:
+    def __init__(self, geom, inner, boundary, degree, bcs=None, g=9.81, threshold_wetdry=0.1,
+                 viscosity=0.0):
+        self.geom = np.ascontiguousarray(geom, np.float64)
+        self.inner = np.ascontiguousarray(inner, np.int32).astype(np.intc)
+        self.boundary = np.ascontiguousarray(boundary, np.int32).astype(np.intc)
+        self.degree = degree
+        self.n1 = degree + 1
+        self.nd = self.n1 ** 2
+        self.nq = (3 * degree) // 2 + 1
+        self.ng = degree + 2
+        self.N = self.geom.shape[0]
+        kinds = np.zeros(256, np.intc)
+        for bid, k in (bcs or {}).items():
+            kinds[bid] = BC[k] if isinstance(k, str) else int(k)
+        prm = np.array([g, threshold_wetdry, viscosity])
+        self._h = lib().ocr_create(degree, self.N, _d(self.geom), self.inner.shape[0],
+                                   self.inner.ctypes.data_as(PI), self.boundary.shape[0],
+                                   self.boundary.ctypes.data_as(PI), kinds.ctypes.data_as(PI),
+                                   _d(prm))
+        if not self._h:
+            raise RuntimeError(lib().ocr_last_error().decode())
+
+    @classmethod
+    def from_mesh(cls, mesh, degree, **kw):
+        return cls(mesh.geom, mesh.inner, mesh.boundary, degree, **kw)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ocr_destroy(self._h)
+            self._h = None
+
+    def _check(self, rc, where):
+        if rc != 0:
+            raise RuntimeError("%s: %s" % (where, lib().ocr_last_error().decode()))
+
+    def quadrature_points(self, store):
+        n = {"cell_0": self.N * self.nq ** 2, "cell_1": self.N * self.ng ** 2,
+             "cell_2": self.N * self.nd, "face": self.inner.shape[0] * self.ng * 2,
+             "boundary": self.boundary.shape[0] * self.nq}[store]
+        xy = np.zeros((n, 2))
+        self._check(lib().ocr_quadrature_points(self._h, STORE[store], _d(xy)), "points")
+        return xy
+
+    def set_static(self, store, values):
+        v = np.ascontiguousarray(values, np.float64)
+        self._check(lib().ocr_set_static(self._h, STORE[store], _d(v)), "set_static")
+
+    def initialize_data_at_quadrature(self, problem_data):
+        data = {"cell_0": problem_data(self.quadrature_points("cell_0")),
+                "cell_1": problem_data(self.quadrature_points("cell_1"))[:, 0],
+                "cell_2": problem_data(self.quadrature_points("cell_2"))[:, :2],
+                "face": problem_data(self.quadrature_points("face"))[:, 0],
+                "boundary": problem_data(self.quadrature_points("boundary"))[:, 0]}
+        for k, v in data.items():
+            self.set_static(k, v)
+        return data
+
+    def set_boundary_values(self, v):
+        self.set_static("bvals", np.ascontiguousarray(v, np.float64).reshape(-1))
+
+    def set_lane_width(self, w):
+        lib().ocr_set_lane_width(self._h, w)
+
+    def set_max_iterations(self, k):
+        lib().ocr_set_max_iterations(self._h, k)
+
+    def last_sweeps(self):
+        return lib().ocr_last_sweeps(self._h)
+
+    def zeros(self, ncomp):
+        return np.zeros(self.N * ncomp * self.nd)
+
+    def rhs(self, h, q, v=None):
+        v = np.zeros_like(q) if v is None else v
+        Lh, Lq = self.zeros(1), self.zeros(2)
+        self._check(lib().ocr_rhs(self._h, _d(h), _d(q), _d(v), _d(Lh), _d(Lq)), "rhs")
+        return Lh, Lq
+
+    def cell_mass_height(self, src, dst):
+        lib().ocr_cell_mass_height(self._h, _d(src), _d(dst))
+
+    def inverse_mass_height(self, dst, src):
+        lib().ocr_inverse_mass_height(self._h, _d(dst), _d(src))
+
+    def inverse_mass_tensor(self, x, ncomp):
+        out = np.zeros_like(x)
+        lib().ocr_inverse_mass_tensor(self._h, _d(x), _d(out), ncomp)
+        return out
+
+    def perform_stage_hydro(self, stage, factor, ri_h, ri_q, ri_v, ki_h, ki_q, w_h, w_q, r_h,
+                            r_q):
+        n_ki = len(ki_h)
+        fac = np.ascontiguousarray(factor, np.float64)
+        ri_v = np.zeros_like(ri_q) if ri_v is None else ri_v
+        kh = (PD * n_ki)(*[_d(k) for k in ki_h])
+        kq = (PD * n_ki)(*[_d(k) for k in ki_q])
+        self._check(lib().ocr_perform_stage_hydro(self._h, stage, n_ki, _d(fac), _d(ri_h),
+                                                  _d(ri_q), _d(ri_v), kh, kq, _d(w_h), _d(w_q),
+                                                  _d(r_h), _d(r_q)), "perform_stage_hydro")
+
+    def project(self, values, ncomp):
+        v = np.ascontiguousarray(values, np.float64)
+        out = self.zeros(ncomp)
+        lib().ocr_project_collocation(self._h, ncomp, _d(v), _d(out))
+        return out
+
+    def project_hydro(self, function, zb_cell2):
+        xy = self.quadrature_points("cell_2")
+        w = function(xy)
+        h = np.maximum(w[:, 0] + zb_cell2, 1e-12)
+        qv = np.ascontiguousarray(np.stack([w[:, 1] * h, w[:, 2] * h], 1))
+        return self.project(w[:, 0].copy(), 1), self.project(qv, 2)
+
+    def errors(self, zeta, q, exact_fn, t):
+        ex = np.ascontiguousarray(exact_fn(self.quadrature_points("cell_0"), t))
+        err = np.zeros(7)
+        lib().ocr_errors(self._h, _d(zeta), _d(q), _d(ex), _d(err))
+        return err
+
+    def time_step(self, scheme_rows, w_h, w_q, ki_h, ki_q, r_h, r_q, v=None):
+        """One explicit RK step with the reference integrator's register usage."""
+        S = len(scheme_rows)
+        for s in range(S):
+            if s == 0:
+                ri_h, ri_q = w_h.copy(), w_q.copy()
+            else:
+                ri_h, ri_q = r_h.copy(), r_q.copy()
+            self.perform_stage_hydro(s, scheme_rows[s][:s + 1], ri_h, ri_q, v, ki_h, ki_q,
+                                     w_h, w_q, r_h, r_q)
