@@ -1,0 +1,179 @@
+"""Throughput of the per-stage DG right-hand side on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): travelling vortex at DG
+degree 3 on the seeded non-conforming mesh -- [0,2]x[0,1], 20x10 coarse cells refined
+globally 4 times, then 30% of the cells (std::mt19937_64 seed 20261202) refined once more:
+97,280 cells with 85,944 hanging subfaces, 4.67 M scalar DoF (zeta, hu, hv), periodic.
+A "step" is one explicit RK stage (oc_dg_perform_stage_hydro) of classic RK4 over the
+whole mesh; inputs are resident in HBM when timing starts.  The metric counts
+DoF x stages / second.
+
+Extra fields: `roofline` for the kernel that takes most of the stage time (algorithmic
+HBM bytes per launch / mean launch time from HIP events on the launch stream, against
+8 TB/s); `cpu_baseline` times the CPU restatement of the same stage (single core) on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "vortex_p3_hanging_97k"
+
+
+def build_problem():
+    from oceano_amd import problems
+    return problems.vortex(degree=3, refine=4, refine_fraction=0.3, seed=20261202)
+
+
+def algorithmic_bytes(op, stage, n_stages, static):
+    """Compulsory HBM bytes of each kernel of one stage (FP64 vectors, int32 topology).
+
+    height_rhs:    reads zeta_r, q_r; writes L_zeta
+    discharge:     reads zeta_r, q_r, w_q, k_q[1..stage]; writes k_q[stage+1], r_q (or w_q)
+    height_jacobi: reads k_h[1..stage+1], w_h; writes r_h (or w_h)
+    plus, per cell, the face tables (4 kinds + 8 neighbours + 8 face ids, int32) for the
+    two face kernels, the cell sizes (2 doubles) for all three, and the stored static
+    fields (none for the vortex: every problem-data field is uniform and not stored)."""
+    n, nd = op.n_cells, op.dofs_per_cell
+    v1, v2 = 8 * nd * n, 16 * nd * n
+    topo = (4 + 8 + 8) * 4 * n
+    geom = 16 * n
+    rhs = v1 + v2 + v1 + topo + geom + static.get("face", 0) + static.get("boundary", 0)
+    dis = v1 + v2 + v2 + stage * v2 + v2 + v2 + topo + geom + static.get("cell_0", 0)
+    jac = (stage + 1) * v1 + v1 + v1 + geom + static.get("cell_1", 0)
+    return {"height_rhs": rhs, "discharge": dis, "height_jacobi": jac}
+
+
+def cpu_baseline(problem, n_stages):
+    """Single-core CPU restatement (oracle/) on the first n_stages RK4 stages."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from oceano_amd.integrator import butcher_rows
+    o = pyoracle.Oracle.from_mesh(problem.mesh, problem.degree, bcs=problem.bcs)
+    d = o.initialize_data_at_quadrature(problem.problem_data)
+    h, q = o.project_hydro(problem.initial, d["cell_2"][:, 0])
+    rows, _ = butcher_rows("stage_4_order_4", 1e-5)
+    S = len(rows)
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(S + 1)]
+    w_h, w_q, r_h, r_q = h.copy(), q.copy(), o.zeros(1), o.zeros(2)
+    t0 = time.perf_counter()
+    for k in range(n_stages):
+        s = k % S
+        ri_h, ri_q = (w_h.copy(), w_q.copy()) if s == 0 else (r_h.copy(), r_q.copy())
+        o.perform_stage_hydro(s, rows[s][:s + 1], ri_h, ri_q, None, ki_h, ki_q, w_h, w_q, r_h,
+                              r_q)
+    dt = time.perf_counter() - t0
+    dofs = problem.dofs()
+    return {"value": dofs * n_stages / dt, "unit": "DoF-RK-stages/s", "cores": 1,
+            "kind": "port",
+            "sample": "%d RK4 stages of the same workload (%d cells, %d DoF) from the "
+                      "projected initial state, %.1f s" % (n_stages, problem.mesh.n_cells,
+                                                          dofs, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200, help="timed RK stages")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dt", type=float, default=1e-5)
+    ap.add_argument("--cpu-stages", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from oceano_amd import problems
+    from oceano_amd.integrator import butcher_rows
+
+    P = build_problem()
+    op, data, w_h, w_q = problems.setup_operator(P)
+    rows, _ = butcher_rows("stage_4_order_4", args.dt)
+    S = len(rows)
+    ki_h = [op.initialize_vector(0) for _ in range(S + 1)]
+    ki_q = [op.initialize_vector(1) for _ in range(S + 1)]
+    r_h = [op.initialize_vector(0) for _ in range(2)]
+    r_q = [op.initialize_vector(1) for _ in range(2)]
+
+    def stage(k):
+        s = k % S
+        ri = [w_h, w_q] if s == 0 else [r_h[s % 2], r_q[s % 2]]
+        op.perform_stage_hydro(s, 0.0, rows[s][:s + 1], ri + [None, None], ki_h, ki_q, w_h,
+                               w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
+
+    for k in range(args.warmup):
+        stage(k)
+    torch.cuda.synchronize()
+    op.set_timing(True)
+    k0 = args.warmup - args.warmup % S  # start the timed region on stage 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        stage(k0 + k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    rep = op.timing_report()
+    op.set_timing(False)
+    sweeps = op.last_sweeps()
+    nonfinite = op.count_nonfinite(w_h) + op.count_nonfinite(w_q)
+
+    dofs = P.dofs()
+    value = dofs * args.steps / elapsed
+    static = {}  # vortex: all problem-data fields uniform, none stored
+    per_stage = [algorithmic_bytes(op, s, S, static) for s in range(S)]
+    counts = {k: 0 for k in rep}
+    for k in range(args.steps):
+        for name in counts:
+            counts[name] += 1
+    kern = {}
+    for name, (ms, n) in rep.items():
+        b = sum(per_stage[(k0 + k) % S][name] for k in range(args.steps)) / max(args.steps, 1)
+        mean_ms = ms / max(n, 1)
+        kern[name] = {"mean_ms": mean_ms, "launches": n, "alg_bytes": b,
+                      "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
+    dom = max(kern, key=lambda k: kern[k]["mean_ms"])
+    achieved = kern[dom]["alg_GBs"]
+    line = {
+        "metric": "DoF-RK-stages/s on 1 GPU, mixed vs FP64 at matched L2 error; % HBM peak",
+        "value": value,
+        "unit": "DoF-RK-stages/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "cells": P.mesh.n_cells,
+                   "hanging_subfaces": P.mesh.n_hanging, "dofs": dofs, "degree": 3,
+                   "scheme": "classic RK4 (Butcher), stage = step", "dt": args.dt,
+                   "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": None},
+        "kernels": kern,
+        "jacobi_sweeps_last_stage": sweeps,
+        "nonfinite": nonfinite,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(P, args.cpu_stages)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -45,6 +45,10 @@ struct oc_dg
   unsigned long long *d_count = nullptr;
   std::vector<double> cell2_host;
   int xcd_swizzle = 1;
+  // optional per-kernel event timing: kernel id -> recorded (start, stop) pairs
+  int timing = 0;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
+  std::vector<hipEvent_t> event_pool;
 
   OpArgs args() const
   {
@@ -100,6 +104,17 @@ namespace oceano {
   int rhs_apply_d##P(oc_dg *op, const OpArgs &O, bool invert, const double *src_h,          \
                      const double *src_q, const double *src_v, double *dst_h,              \
                      double *dst_q, hipStream_t s);
+// kernels timed by oc_dg_set_timing
+enum TimedKernel
+{
+  TK_HEIGHT_RHS = 0,
+  TK_DISCHARGE = 1,
+  TK_HEIGHT_JACOBI = 2,
+  TK_COUNT = 3
+};
+int timed_begin(oc_dg *op, hipStream_t s, hipEvent_t *start);
+void timed_end(oc_dg *op, hipStream_t s, int kernel, hipEvent_t start);
+
 OC_DECLARE_DEGREE(1)
 OC_DECLARE_DEGREE(2)
 OC_DECLARE_DEGREE(3)

@@ -41,9 +41,12 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
 {
   const bool last = (stage == n_ki - 2);
   const dim3 grid(op->NB), block(CB);
+  hipEvent_t ev = nullptr;
   // 1) L_zeta -> ki_h[stage+1]
+  timed_begin(op, s, &ev);
   hipLaunchKernelGGL((k_height_rhs<P, false>), grid, block, 0, s, O, ri_h, ri_q,
                      ki_h[stage + 1]);
+  timed_end(op, s, TK_HEIGHT_RHS, ev);
   // 2) L_q -> M^-1 L_q -> ki_q[stage+1], and r_q (or w_q) = w_q + sum_j a_j ki_q[j+1]
   double *q_dst = last ? w_q : r_q;
   const bool alias = (q_dst == ri_q);  // only for single-stage schemes (forward Euler)
@@ -61,6 +64,7 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
   S.wsol = w_q;
   S.rk_dst = q_dst;
   S.sweeps = nullptr;
+  timed_begin(op, s, &ev);
   if (alias)
     {
       // single-stage schemes: ri_q is the updated vector itself, so the inverse mass must
@@ -86,6 +90,7 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
     hipLaunchKernelGGL((k_discharge<P, true, 0>), grid, block, 0, s, O, S);
   else
     hipLaunchKernelGGL((k_discharge<P, false, 0>), grid, block, 0, s, O, S);
+  timed_end(op, s, TK_DISCHARGE, ev);
   // 3) front and Jacobi depth solve -> r_h (or w_h in place at the last stage)
   StageArgs H{};
   for (uint32_t j = 0; j <= stage; ++j)
@@ -99,7 +104,9 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
   H.scratch = write_scratch ? ki_h[0] : nullptr;
   H.sweeps = op->d_sweeps;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, op->d_sweeps);
+  timed_begin(op, s, &ev);
   hipLaunchKernelGGL((k_height_jacobi<P>), grid, block, 0, s, O, H);
+  timed_end(op, s, TK_HEIGHT_JACOBI, ev);
   HIP_TRY(hipGetLastError());
   return OC_OK;
 }

@@ -169,6 +169,43 @@ __global__ void k_count_nonfinite(const double *__restrict__ x, size_t n,
 }
 
 }  // namespace
+
+static hipEvent_t pool_event(oc_dg *op)
+{
+  if (!op->event_pool.empty())
+    {
+      hipEvent_t e = op->event_pool.back();
+      op->event_pool.pop_back();
+      return e;
+    }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess)
+    return nullptr;
+  return e;
+}
+
+int timed_begin(oc_dg *op, hipStream_t s, hipEvent_t *start)
+{
+  *start = nullptr;
+  if (!op->timing)
+    return OC_OK;
+  *start = pool_event(op);
+  if (*start)
+    (void)hipEventRecord(*start, s);
+  return OC_OK;
+}
+
+void timed_end(oc_dg *op, hipStream_t s, int kernel, hipEvent_t start)
+{
+  if (!op->timing || !start)
+    return;
+  hipEvent_t stop = pool_event(op);
+  if (!stop)
+    return;
+  (void)hipEventRecord(stop, s);
+  op->events.push_back({kernel, {start, stop}});
+}
+
 }  // namespace oceano
 
 using namespace oceano;
@@ -177,6 +214,13 @@ namespace {
 
 void free_all(oc_dg *op)
 {
+  for (auto &e : op->events)
+    {
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+  for (auto e : op->event_pool)
+    (void)hipEventDestroy(e);
   auto f = [](void *p) {
     if (p)
       (void)hipFree(p);
@@ -849,6 +893,37 @@ int oc_dg_count_nonfinite(oc_dg *op, const double *x, int64_t n, int64_t *count,
   HIP_TRY(hipMemcpyAsync(&h, op->d_count, sizeof(h), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   *count = (int64_t)h;
+  return OC_OK;
+}
+
+int oc_dg_set_timing(oc_dg *op, int32_t enable)
+{
+  if (!op)
+    return OC_ERR_ARGUMENT;
+  op->timing = enable != 0;
+  return OC_OK;
+}
+
+int oc_dg_timing_report(oc_dg *op, double *ms, int64_t *count)
+{
+  if (!op || !ms || !count)
+    return OC_ERR_ARGUMENT;
+  for (int k = 0; k < TK_COUNT; ++k)
+    {
+      ms[k] = 0.0;
+      count[k] = 0;
+    }
+  for (auto &e : op->events)
+    {
+      HIP_TRY(hipEventSynchronize(e.second.second));
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, e.second.first, e.second.second));
+      ms[e.first] += t;
+      count[e.first] += 1;
+      op->event_pool.push_back(e.second.first);
+      op->event_pool.push_back(e.second.second);
+    }
+  op->events.clear();
   return OC_OK;
 }
 

@@ -87,6 +87,11 @@ def test_rk4_step(cuda, case, degree):
     _print_case(case, degree)
     P = CASES[case](degree)
     op, o, h, q = make_pair(P)
+    # a generic state: the projected initial states are near-steady (channel) or balanced
+    # (vortex), where L is mostly cancellation and its relative error says little
+    rng = np.random.default_rng(100 + degree)
+    h = h + 1e-3 * rng.standard_normal(h.shape)
+    q = q + 1e-2 * rng.standard_normal(q.shape)
     hmin = float(np.min(P.mesh.geom[:, 2:]))
     dt = 0.1 * hmin / (8.0 * (2 * degree + 1))
     rows, _ = butcher_rows("stage_4_order_4", dt)
@@ -115,37 +120,67 @@ def test_vortex_many_steps_hanging(cuda):
     assert eh <= TOL and eq <= TOL and eh_d <= 1e-9
 
 
-def _wetdry_problem(degree):
-    """Sloping beach: dry for x > 0.55 at rest level 0, a mound of water on the wet side."""
+def _island_problem(degree):
+    """Sea over an island whose top emerges inside a few cells: partially dry cells, so the
+    masked lumped mass and the dry-point count of the depth solve are exercised."""
     from oceano_amd.mesh import Mesh
     mesh = Mesh.rectangle(0.0, 1.0, 0.0, 0.5, 12, 6, boundary_ids=(0, 0, 0, 0))
 
     def data(xy):
         out = np.zeros((xy.shape[0], 5))
-        out[:, 0] = 0.55 - xy[:, 0]
+        x, y = xy[:, 0], xy[:, 1]
+        out[:, 0] = 0.1 - 0.3 * np.exp(-((x - 0.4583) ** 2 + (y - 0.25) ** 2) / 0.005)
         out[:, 1] = 0.02
         return out
 
     def initial(xy):
         x, y = xy[:, 0], xy[:, 1]
-        zeta = 0.05 * np.exp(-((x - 0.3) ** 2 + (y - 0.25) ** 2) / 0.01)
         zb = data(xy)[:, 0]
-        zeta = np.maximum(zeta, -zb)  # dry points sit on the bed
-        return np.stack([zeta, 0.1 * np.ones_like(x), np.zeros_like(x)], 1)
+        # the free surface passes below the island top: dry points have h clearly < 0, so
+        # no wet/dry decision sits on the threshold where rounding would pick the branch
+        zeta = np.maximum(0.01 * np.exp(-((x - 0.25) ** 2 + (y - 0.25) ** 2) / 0.01),
+                          -zb - 0.02)
+        return np.stack([zeta, 0.05 * (zb > 0), np.zeros_like(x)], 1)
 
-    return problems.Problem(mesh, degree, {0: "wall"}, data, initial, name="beach")
+    return problems.Problem(mesh, degree, {0: "wall"}, data, initial, name="island")
 
 
-@pytest.mark.parametrize("degree", [1, 2, 3])
-def test_wet_dry_stage(cuda, degree):
-    """Dry lift, masked lumped mass and the l1 stop test of the depth solve."""
-    P = _wetdry_problem(degree)
+def _dambreak_problem(degree):
+    """Dam break onto a dry flat bed: fully dry cells take the dry lift
+    (OceanDG.h:1399-1422) and are then flooded; the sweep count varies per cell."""
+    from oceano_amd.mesh import Mesh
+    mesh = Mesh.rectangle(0.0, 1.0, 0.0, 0.5, 12, 6, boundary_ids=(0, 0, 0, 0))
+
+    def data(xy):
+        out = np.zeros((xy.shape[0], 5))
+        out[:, 0] = 0.2
+        out[:, 1] = 0.02
+        return out
+
+    def initial(xy):
+        x = xy[:, 0]
+        return np.stack([np.where(x < 0.5, 0.0, -0.3), np.zeros_like(x), np.zeros_like(x)], 1)
+
+    return problems.Problem(mesh, degree, {0: "wall"}, data, initial, name="dambreak")
+
+
+# The reference's lumped depth solve divides by D_i = integral(phi_i * wet mask); when all
+# quadrature points that support phi_i are dry in a partially wet cell, D_i = 0 and the
+# reference arithmetic yields inf/NaN (test_partially_dry_degenerate_lumped_mass).  For
+# the island that happens at p >= 2 within a few stages, so these cases are those where
+# the reference stays finite.
+@pytest.mark.parametrize("maker,degree", [(_island_problem, 1), (_dambreak_problem, 1),
+                                          (_dambreak_problem, 2), (_dambreak_problem, 3)])
+def test_wet_dry(cuda, maker, degree):
+    P = maker(degree)
+    print("[parity] case=%s degree=%d" % (P.name, degree), flush=True)
     op, o, h, q = make_pair(P)
-    rows, _ = butcher_rows("stage_4_order_4", 1e-3)
-    (gw_h, gw_q, gki_h, _), (w_h, w_q, ki_h, _) = _run_steps(op, o, h, q, rows, 2)
+    rows, _ = butcher_rows("stage_4_order_4", 2e-3)
+    (gw_h, gw_q, gki_h, _), (w_h, w_q, ki_h, _) = _run_steps(op, o, h, q, rows, 3)
     eh, eq = rel_l2(to_host(op, gw_h, 1), w_h), rel_l2(to_host(op, gw_q, 2), w_q)
     print("  wet/dry: zeta %.2e q %.2e sweeps gpu %d oracle %d" %
           (eh, eq, op.last_sweeps(), o.last_sweeps()))
+    assert np.isfinite(w_h).all() and np.isfinite(w_q).all()
     assert eh <= TOL and eq <= TOL
     assert op.last_sweeps() == o.last_sweeps()
 

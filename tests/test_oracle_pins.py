@@ -1,0 +1,276 @@
+"""Pins of the CPU oracle (oracle/oracle.cpp) against mathematics and invariants.
+
+The reference ships no golden vectors or known-answer tests for this path (its tests/
+holds two run decks), and deal.II -- which holds the quadrature, basis, sum-factorisation
+and inverse-mass arithmetic -- is not available, so every piece is pinned here by closed
+forms, library routines (numpy) and invariants chosen so that a dropped term, a wrong
+sign/index or a transposed operand fails one of them.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from oceano_amd import problems
+from oceano_amd.integrator import butcher_rows
+from oceano_amd.mesh import Mesh
+
+
+# ----------------------------------------------------------------- quadrature -------
+@pytest.mark.parametrize("n", range(1, 9))
+def test_gauss_matches_numpy(n):
+    x, w = po.rule("gauss", n)
+    xn, wn = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(x, (xn + 1) / 2, atol=1e-15, rtol=0)
+    assert np.allclose(w, wn / 2, atol=1e-15, rtol=0)
+
+
+@pytest.mark.parametrize("n", range(2, 9))
+def test_gauss_lobatto_nodes_weights_exactness(n):
+    x, w = po.rule("lobatto", n)
+    assert x[0] == 0.0 and x[-1] == 1.0
+    # interior nodes are the roots of P'_{n-1}
+    c = np.zeros(n)
+    c[-1] = 1
+    roots = np.sort((np.polynomial.legendre.legroots(np.polynomial.legendre.legder(c)) + 1) / 2)
+    assert np.allclose(x[1:-1], roots, atol=1e-14)
+    # exact for x^k, k <= 2n-3, not for 2n-2
+    for k in range(2 * n - 2):
+        assert abs(np.dot(w, x ** k) - 1.0 / (k + 1)) < 1e-14
+    k = 2 * n - 2
+    assert abs(np.dot(w, x ** k) - 1.0 / (k + 1)) > 1e-10
+
+
+# ----------------------------------------------------------------- basis -------------
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_lagrange_identities(p):
+    nodes, _ = po.rule("lobatto", p + 1)
+    V, D = po.lagrange(nodes, nodes)
+    assert np.allclose(V, np.eye(p + 1), atol=1e-15)
+    pts = np.linspace(0.03, 0.97, 11)
+    V, D = po.lagrange(nodes, pts)
+    assert np.allclose(V.sum(1), 1.0, atol=1e-14)       # partition of unity
+    assert np.allclose(D.sum(1), 0.0, atol=1e-12)
+    for k in range(p + 1):                               # exact for degree <= p
+        assert np.allclose(V @ nodes ** k, pts ** k, atol=1e-13)
+        assert np.allclose(D @ nodes ** k, k * pts ** max(k - 1, 0) * (k > 0), atol=1e-12)
+
+
+# ------------------------------------------------------ single-cell brute force -------
+def _single_cell_oracle(p, hx=0.37, hy=0.21):
+    geom = np.array([[0.1, 0.2, hx, hy]])
+    inner = np.array([[0, 1, -1, 0, 0, -1], [0, 3, -1, 0, 2, -1]], np.int32)  # periodic
+    return po.Oracle(geom, inner, np.zeros((0, 3), np.int32), p)
+
+
+def _dense_mass(p, hx, hy, rule_n):
+    nodes, _ = po.rule("lobatto", p + 1)
+    x, w = rule_n
+    V, _ = po.lagrange(nodes, x)
+    M1 = (V * w[:, None]).T @ V
+    return np.kron(M1 * hy, M1 * hx)  # index i + n1*j: kron(y, x)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_inverse_mass_is_dense_inverse(p):
+    """CellwiseInverseMassMatrix == inverse of the exactly integrated mass matrix."""
+    hx, hy = 0.37, 0.21
+    o = _single_cell_oracle(p, hx, hy)
+    M = _dense_mass(p, hx, hy, po.rule("gauss", p + 4))  # exact for degree 2p
+    rng = np.random.default_rng(p)
+    x = rng.standard_normal((p + 1) ** 2)
+    y = o.inverse_mass_tensor(M @ x, 1)
+    assert np.allclose(y, x, rtol=0, atol=1e-12 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_depth_mass_is_dense_mass(p):
+    """cell_mass_height with z_b = 0 and a wet state is M_GL zeta (GL(p+2) is exact)."""
+    hx, hy = 0.37, 0.21
+    o = _single_cell_oracle(p, hx, hy)
+    o.initialize_data_at_quadrature(lambda xy: np.zeros((len(xy), 5)))
+    M = _dense_mass(p, hx, hy, po.rule("lobatto", p + 2))
+    Mex = _dense_mass(p, hx, hy, po.rule("gauss", p + 4))
+    assert np.allclose(M, Mex, atol=1e-15)
+    z = 1.0 + 0.1 * np.random.default_rng(0).standard_normal((p + 1) ** 2)
+    out = np.zeros_like(z)
+    o.cell_mass_height(z, out)
+    assert np.allclose(out, M @ z, atol=1e-15)
+
+
+# ------------------------------------------------------------ depth-mass Jacobi -------
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_jacobi_fixed_point(p):
+    """front = M h(zeta): the first sweep changes nothing and the loop stops after it."""
+    P = problems.lake_at_rest(degree=p)
+    o = po.Oracle.from_mesh(P.mesh, p, bcs=P.bcs)
+    o.initialize_data_at_quadrature(P.problem_data)
+    z = 1.0 + 0.01 * np.random.default_rng(1).standard_normal(o.N * o.nd)
+    front = o.zeros(1)
+    o.cell_mass_height(z, front)
+    zz = z.copy()
+    o.inverse_mass_height(zz, front)
+    assert np.abs(zz - z).max() < 1e-13
+    assert o.last_sweeps() == o.N  # one sweep per cell
+
+
+# spectral radius of I - D^-1 M for the lumped GL(p+2) mass (documented for the reference)
+RHO = {1: 0.889, 2: 0.840, 3: 0.816, 4: 0.803}
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_jacobi_matches_dense_iteration(p):
+    """15 capped sweeps equal the dense linear Jacobi iteration on a wet cell; its
+    iteration matrix has the spectral radius quoted for the reference's scheme."""
+    hx, hy = 0.37, 0.21
+    o = _single_cell_oracle(p, hx, hy)
+    o.initialize_data_at_quadrature(lambda xy: np.zeros((len(xy), 5)))
+    M = _dense_mass(p, hx, hy, po.rule("lobatto", p + 2))
+    D = M.sum(1)  # lumped: row sums (integral of phi_i)
+    Jm = np.eye(len(D)) - M / D[:, None]
+    rho = max(abs(np.linalg.eigvals(Jm)))
+    assert abs(rho - RHO[p]) < 1e-3
+    rng = np.random.default_rng(p)
+    z0 = 1.0 + 0.01 * rng.standard_normal(len(D))
+    target = z0 + 1e-3 * rng.standard_normal(len(D))
+    front = M @ target
+    z = z0.copy()
+    for _ in range(15):
+        z = z + (front - M @ z) / D
+    zz = z0.copy()
+    o.inverse_mass_height(zz, front)
+    assert o.last_sweeps() == 15
+    assert np.allclose(zz, z, atol=1e-14)
+    # and it is deliberately not converged: the remaining error follows rho^15
+    assert np.abs(zz - target).max() > 1e-3 * 1e-3
+
+
+def test_dry_lift():
+    """An all-dry cell restarts from -max(z_b) + 1e-9 (OceanDG.h:1399-1422); with the
+    front of that lifted state the first sweep is a fixed point."""
+    o = _single_cell_oracle(1)
+    o.set_static("cell_1", np.full(9, -0.2))
+    lift = -(-0.2) + 1e-09
+    front = np.zeros(4)
+    o.cell_mass_height(np.full(4, lift), front)
+    zz = np.full(4, 0.1)          # below the bed everywhere: h = 0.1 - 0.2 < 0
+    o.inverse_mass_height(zz, front)
+    assert np.array_equal(zz, np.full(4, lift))
+    assert o.last_sweeps() == 1
+
+
+def test_partially_dry_degenerate_lumped_mass():
+    """Reference behaviour kept as is: when every quadrature point supporting a basis
+    function is dry in a partially wet cell, the masked lumped mass D_i is 0 and the
+    update 1/D_i * rhs_i is not finite (operators.h:294-297 has no guard)."""
+    o = _single_cell_oracle(1)
+    zb = np.array([-0.3, -0.25, -0.2, -0.31, -0.26, -0.21, -0.32, -0.27, -0.22])
+    o.set_static("cell_1", zb)
+    zz = np.full(4, 0.1)
+    o.inverse_mass_height(zz, np.zeros(4))
+    assert not np.isfinite(zz).all()
+
+
+# ------------------------------------------------------------ fluxes ------------------
+def _rhs_two_cells(state_m, state_p, p=1, zb=(0.0, 0.0)):
+    """Two cells side by side, periodic in y, walls nowhere: returns L of cell 0 / 1."""
+    geom = np.array([[0.0, 0.0, 1.0, 1.0], [1.0, 0.0, 1.0, 1.0]])
+    inner = np.array([[0, 1, -1, 1, 0, -1], [1, 1, -1, 0, 0, -1],
+                      [0, 3, -1, 0, 2, -1], [1, 3, -1, 1, 2, -1]], np.int32)
+    o = po.Oracle(geom, inner, np.zeros((0, 3), np.int32), p)
+    o.initialize_data_at_quadrature(
+        lambda xy: np.stack([np.where(xy[:, 0] < 1.0, zb[0], zb[1])] + [0 * xy[:, 0]] * 4, 1))
+    nd = (p + 1) ** 2
+    h = np.concatenate([np.full(nd, state_m[0]), np.full(nd, state_p[0])])
+    q = np.concatenate([np.full(nd, state_m[1]), np.full(nd, state_m[2]),
+                        np.full(nd, state_p[1]), np.full(nd, state_p[2])])
+    return o.rhs(h, q)
+
+
+def test_flux_conservation_and_upwinding():
+    """Piecewise-constant states: cell integrals vanish, so L is the face flux.  Mass is
+    conserved exactly, and the LLF flux reduces to the physical flux for equal states."""
+    g = 9.81
+    Lh, Lq = _rhs_two_cells((1.2, 0.3, 0.1), (0.9, -0.2, 0.05))
+    assert abs(Lh.sum()) < 1e-14
+    # equal states on both sides: no net face contribution anywhere
+    Lh, Lq = _rhs_two_cells((1.0, 0.3, 0.1), (1.0, 0.3, 0.1))
+    assert np.abs(Lh).max() < 1e-14 and np.abs(Lq).max() < 1e-14
+    # closed form for one face: zeta jump only, q = 0 -> F = 0.5*lambda*(z_m - z_p)
+    Lh, Lq = _rhs_two_cells((1.2, 0.0, 0.0), (1.0, 0.0, 0.0))
+    lam = max(np.sqrt(g * 1.2), np.sqrt(g * 1.0))
+    F = 0.5 * lam * (1.2 - 1.0)
+    # p = 1, unit cells: the face x=1 carries nodes 1, 3 of cell 0 (minus side, gets -F)
+    # and nodes 0, 2 of cell 1 (plus side, gets +F); each node's face weight is 1/2
+    nodes = Lh.reshape(2, 4)
+    assert abs((nodes[0, 1] + nodes[0, 3]) + F) < 1e-13
+    assert abs((nodes[1, 0] + nodes[1, 2]) - F) < 1e-13
+    # pressure: centred strong jump P = g/4 (h_p + h_m)(zeta_p - zeta_m) n, submitted with
+    # the same sign on both sides (-P on the minus side, -P on the plus side)
+    P = g * 0.25 * (1.0 + 1.2) * (1.0 - 1.2)
+    qx = Lq.reshape(2, 2, 4)[:, 0]
+    assert abs((qx[0, 1] + qx[0, 3]) + P) < 1e-13
+    assert abs((qx[1, 0] + qx[1, 2]) + P) < 1e-13
+    assert np.abs(Lq.reshape(2, 2, 4)[:, 1]).max() < 1e-14  # no y-momentum
+
+
+# ------------------------------------------------------- whole-operator invariants -----
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("frac", [None, 0.3])
+def test_free_stream_and_mass(p, frac):
+    m = Mesh.rectangle(0, 2, 0, 1, 8, 4, periodic=(True, True)).refine_global(1)
+    if frac:
+        m.refine_fraction(frac, 5)
+    o = po.Oracle.from_mesh(m, p)
+    o.initialize_data_at_quadrature(lambda xy: np.zeros((len(xy), 5)))
+    h = o.project(np.ones(o.N * o.nd), 1)
+    qv = np.zeros((o.N * o.nd, 2))
+    qv[:, 0], qv[:, 1] = 1.0, 0.5
+    Lh, Lq = o.rhs(h, o.project(qv, 2))
+    assert np.abs(Lh).max() < 1e-14 and np.abs(Lq).max() < 1e-14
+    rng = np.random.default_rng(1)
+    Lh, _ = o.rhs(1 + 0.1 * rng.standard_normal(o.N * o.nd),
+                  0.1 * rng.standard_normal(2 * o.N * o.nd))
+    assert abs(Lh.sum()) < 1e-13
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_lake_at_rest(p):
+    P = problems.lake_at_rest(degree=p)
+    o = po.Oracle.from_mesh(P.mesh, p, bcs=P.bcs)
+    d = o.initialize_data_at_quadrature(P.problem_data)
+    h, q = o.project_hydro(P.initial, d["cell_2"][:, 0])
+    Lh, Lq = o.rhs(h, q)
+    assert np.abs(Lh).max() < 1e-14 and np.abs(Lq).max() < 1e-13
+
+
+def test_projection_error_p1_is_roundoff():
+    """At p = 1 the projection's Gauss(p+1) points are the error quadrature's points, so the
+    t = 0 error is round-off (README.md:71 reports 6.291e-16 / 4.93e-15)."""
+    P = problems.vortex(degree=1, refine=1)
+    o = po.Oracle.from_mesh(P.mesh, 1)
+    d = o.initialize_data_at_quadrature(P.problem_data)
+    h, q = o.project_hydro(P.initial, d["cell_2"][:, 0])
+    err = o.errors(h, q, P.exact, 0.0)
+    assert err[0] < 1e-14 and err[1] < 1e-13
+
+
+def test_vortex_convergence():
+    """The stage reproduces the travelling vortex with the DG order of accuracy."""
+    errs = []
+    for refine in (1, 2):
+        P = problems.vortex(degree=2, refine=refine, u_inf=(1.0, 0.0))
+        o = po.Oracle.from_mesh(P.mesh, 2)
+        d = o.initialize_data_at_quadrature(P.problem_data)
+        h, q = o.project_hydro(P.initial, d["cell_2"][:, 0])
+        dt = 2.5e-4 / refine
+        n = int(round(0.01 / dt))
+        rows, _ = butcher_rows("stage_4_order_4", dt)
+        ki_h = [o.zeros(1) for _ in range(5)]
+        ki_q = [o.zeros(2) for _ in range(5)]
+        r_h, r_q = o.zeros(1), o.zeros(2)
+        for _ in range(n):
+            o.time_step(rows, h, q, ki_h, ki_q, r_h, r_q)
+        errs.append(o.errors(h, q, P.exact, n * dt))
+    order_h = np.log2(errs[0][0] / errs[1][0])
+    order_q = np.log2(errs[0][1] / errs[1][1])
+    assert order_h > 2.3 and order_q > 2.3, (errs, order_h, order_q)
