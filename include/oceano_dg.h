@@ -196,13 +196,14 @@ int oc_dg_count_nonfinite(oc_dg *op, const double *x, int64_t n, int64_t *count,
 int oc_dg_last_sweeps(oc_dg *op, int64_t *sweeps, void *stream);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around each kernel of
- * oc_dg_perform_stage_hydro (0 = L_zeta, 1 = L_q + inverse mass + RK update, 2 = front +
- * depth-mass Jacobi).  oc_dg_timing_report synchronises, returns the summed milliseconds
+ * oc_dg_perform_stage_hydro: 0 = the one-pass stage kernel; on the split path taken when
+ * outputs alias inputs, 1 = L_zeta, 2 = L_q + inverse mass, 3 = discharge RK update,
+ * 4 = front + depth-mass Jacobi.  oc_dg_timing_report synchronises, returns the summed milliseconds
  * and launch counts per kernel since the last report, and clears them.  The reference
  * times the same two parts with TimerOutput sections "rk_stage hydro - integrals L_h" and
  * "rk_stage hydro - inv mass + vec upd" (OceanDG.h:1796, 1829). */
 int oc_dg_set_timing(oc_dg *op, int32_t enable);
-int oc_dg_timing_report(oc_dg *op, double ms[3], int64_t count[3]);
+int oc_dg_timing_report(oc_dg *op, double ms[5], int64_t count[5]);
 
 /* ------------------------------------------------------------------------------------
  * Test-case functions (ICBC layer).  Host memory; evaluate at n points xy[n][2].

@@ -41,7 +41,7 @@ struct oc_dg
   double *d_bz = nullptr;
   double c_bz = 0;
   double *d_bvals = nullptr;
-  unsigned long long *d_sweeps = nullptr;
+  unsigned int *d_sweeps_blk = nullptr;
   unsigned long long *d_count = nullptr;
   std::vector<double> cell2_host;
   int xcd_swizzle = 1;
@@ -107,10 +107,12 @@ namespace oceano {
 // kernels timed by oc_dg_set_timing
 enum TimedKernel
 {
-  TK_HEIGHT_RHS = 0,
-  TK_DISCHARGE = 1,
-  TK_HEIGHT_JACOBI = 2,
-  TK_COUNT = 3
+  TK_STAGE = 0,         // one-pass stage (height and discharge work in one grid)
+  TK_HEIGHT_RHS = 1,    // split path: L_zeta
+  TK_DISCHARGE = 2,     // split path: L_q + inverse mass
+  TK_RK_UPDATE = 3,     // split path: discharge RK combination
+  TK_HEIGHT_JACOBI = 4, // split path: front + depth solve
+  TK_COUNT = 5
 };
 int timed_begin(oc_dg *op, hipStream_t s, hipEvent_t *start);
 void timed_end(oc_dg *op, hipStream_t s, int kernel, hipEvent_t start);

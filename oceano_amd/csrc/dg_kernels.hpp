@@ -78,7 +78,7 @@ struct StageArgs
   int stage;                 // number of previous k vectors = stage
   const double *wsol;        // solution (w_h or w_q)
   double *rk_dst;            // r or w
-  unsigned long long *sweeps;
+  unsigned int *sweeps_blk;  // per-batch sum of Jacobi sweeps (height kernels)
 };
 
 __host__ __device__ constexpr int nq_of(int p) { return (3 * p) / 2 + 1; }
@@ -102,6 +102,14 @@ __device__ __forceinline__ int logical_batch(int b, int nb, int swz)
 __device__ __forceinline__ size_t vix(int batch, int ncomp, int comp, int nd, int k, int lane)
 {
   return (((size_t)batch * ncomp + comp) * nd + k) * CB + lane;
+}
+
+// Keeps the scheduler from hoisting the loads of a later phase (the next face's
+// neighbour traces) above the current one, which would keep them all live at once.
+__device__ __forceinline__ void phase_barrier()
+{
+  __builtin_amdgcn_sched_barrier(0);
+  asm volatile("" ::: "memory");
 }
 
 __device__ __forceinline__ double fld(const Field &f, size_t i)

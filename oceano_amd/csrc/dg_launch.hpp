@@ -7,8 +7,6 @@
 namespace oceano {
 namespace {
 
-__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
-
 struct RkArgs
 {
   size_t n;
@@ -41,71 +39,82 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
 {
   const bool last = (stage == n_ki - 2);
   const dim3 grid(op->NB), block(CB);
+  double *h_dst = last ? w_h : r_h;
+  double *q_dst = last ? w_q : r_q;
+  const bool visc = O.nu != 0.0;
   hipEvent_t ev = nullptr;
-  // 1) L_zeta -> ki_h[stage+1]
+  // discharge: L_q -> M^-1 L_q -> ki_q[stage+1]; q_dst = w_q + sum_j a_j ki_q[j+1]
+  StageArgs Q{};
+  Q.src_h = ri_h;
+  Q.src_q = ri_q;
+  Q.src_v = ri_v;
+  Q.dst = ki_q[stage + 1];
+  Q.scratch = write_scratch ? ki_q[0] : nullptr;
+  for (uint32_t j = 0; j < stage; ++j)
+    Q.kprev[j] = ki_q[j + 1];
+  for (uint32_t j = 0; j <= stage; ++j)
+    Q.factor[j] = factor[j];
+  Q.stage = (int)stage;
+  Q.wsol = w_q;
+  Q.rk_dst = q_dst;
+  // height: L_zeta -> ki_h[stage+1]; front; Jacobi -> h_dst
+  StageArgs H{};
+  H.src_h = ri_h;
+  H.src_q = ri_q;
+  H.dst = ki_h[stage + 1];
+  for (uint32_t j = 0; j < stage; ++j)
+    H.kprev[j] = ki_h[j + 1];
+  for (uint32_t j = 0; j <= stage; ++j)
+    H.factor[j] = factor[j];
+  H.stage = (int)stage;
+  H.wsol = w_h;
+  H.rk_dst = h_dst;
+  H.scratch = write_scratch ? ki_h[0] : nullptr;
+  H.sweeps_blk = op->d_sweeps_blk;
+  // the one-pass stage reads neighbour traces of ri while other cells write the outputs:
+  // it needs outputs distinct from the inputs (single-stage schemes write w in place)
+  const bool alias = (h_dst == ri_h) || (q_dst == ri_q) || (h_dst == ri_q) || (q_dst == ri_h);
+  if (!alias)
+    {
+      timed_begin(op, s, &ev);
+      if (visc)
+        hipLaunchKernelGGL((k_stage<P, true>), dim3(2 * op->NB), block, 0, s, O, H, Q);
+      else
+        hipLaunchKernelGGL((k_stage<P, false>), dim3(2 * op->NB), block, 0, s, O, H, Q);
+      timed_end(op, s, TK_STAGE, ev);
+      HIP_TRY(hipGetLastError());
+      return OC_OK;
+    }
+  // split path: every kernel finishes reading ri before the next one overwrites it
   timed_begin(op, s, &ev);
   hipLaunchKernelGGL((k_height_rhs<P, false>), grid, block, 0, s, O, ri_h, ri_q,
                      ki_h[stage + 1]);
   timed_end(op, s, TK_HEIGHT_RHS, ev);
-  // 2) L_q -> M^-1 L_q -> ki_q[stage+1], and r_q (or w_q) = w_q + sum_j a_j ki_q[j+1]
-  double *q_dst = last ? w_q : r_q;
-  const bool alias = (q_dst == ri_q);  // only for single-stage schemes (forward Euler)
-  StageArgs S{};
-  S.src_h = ri_h;
-  S.src_q = ri_q;
-  S.src_v = ri_v;
-  S.dst = ki_q[stage + 1];
-  S.scratch = write_scratch ? ki_q[0] : nullptr;
-  for (uint32_t j = 0; j < stage; ++j)
-    S.kprev[j] = ki_q[j + 1];
-  for (uint32_t j = 0; j <= stage; ++j)
-    S.factor[j] = factor[j];
-  S.stage = (int)stage;
-  S.wsol = w_q;
-  S.rk_dst = q_dst;
-  S.sweeps = nullptr;
   timed_begin(op, s, &ev);
-  if (alias)
-    {
-      // single-stage schemes: ri_q is the updated vector itself, so the inverse mass must
-      // finish on every cell (neighbour traces) before the update overwrites it
-      StageArgs A = S;
-      if (O.nu != 0.0)
-        hipLaunchKernelGGL((k_discharge<P, true, 1>), grid, block, 0, s, O, A);
-      else
-        hipLaunchKernelGGL((k_discharge<P, false, 1>), grid, block, 0, s, O, A);
-      RkArgs U{};
-      U.n = (size_t)op->NB * CB * 2 * op->nd;
-      for (uint32_t j = 0; j <= stage; ++j)
-        {
-          U.k[j] = ki_q[j + 1];
-          U.factor[j] = factor[j];
-        }
-      U.nk = (int)stage + 1;
-      U.w = w_q;
-      U.dst = q_dst;
-      hipLaunchKernelGGL(k_rk_update, dim3((U.n + 255) / 256), dim3(256), 0, s, U);
-    }
-  else if (O.nu != 0.0)
-    hipLaunchKernelGGL((k_discharge<P, true, 0>), grid, block, 0, s, O, S);
+  StageArgs A = Q;
+  if (visc)
+    hipLaunchKernelGGL((k_discharge<P, true, 1>), grid, block, 0, s, O, A);
   else
-    hipLaunchKernelGGL((k_discharge<P, false, 0>), grid, block, 0, s, O, S);
+    hipLaunchKernelGGL((k_discharge<P, false, 1>), grid, block, 0, s, O, A);
   timed_end(op, s, TK_DISCHARGE, ev);
-  // 3) front and Jacobi depth solve -> r_h (or w_h in place at the last stage)
-  StageArgs H{};
+  RkArgs U{};
+  U.n = (size_t)op->NB * CB * 2 * op->nd;
   for (uint32_t j = 0; j <= stage; ++j)
     {
-      H.kprev[j] = ki_h[j + 1];
-      H.factor[j] = factor[j];
+      U.k[j] = ki_q[j + 1];
+      U.factor[j] = factor[j];
     }
-  H.stage = (int)stage;
-  H.wsol = w_h;
-  H.rk_dst = last ? w_h : r_h;
-  H.scratch = write_scratch ? ki_h[0] : nullptr;
-  H.sweeps = op->d_sweeps;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, op->d_sweeps);
+  U.nk = (int)stage + 1;
+  U.w = w_q;
+  U.dst = q_dst;
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_height_jacobi<P>), grid, block, 0, s, O, H);
+  hipLaunchKernelGGL(k_rk_update, dim3((U.n + 255) / 256), dim3(256), 0, s, U);
+  timed_end(op, s, TK_RK_UPDATE, ev);
+  StageArgs J = H;
+  for (uint32_t j = 0; j <= stage; ++j)
+    J.kprev[j] = ki_h[j + 1];
+  timed_begin(op, s, &ev);
+  hipLaunchKernelGGL((k_height_jacobi<P>), grid, block, 0, s, O, J);
   timed_end(op, s, TK_HEIGHT_JACOBI, ev);
   HIP_TRY(hipGetLastError());
   return OC_OK;

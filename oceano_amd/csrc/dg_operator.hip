@@ -236,7 +236,7 @@ void free_all(oc_dg *op)
   f(op->d_face);
   f(op->d_bz);
   f(op->d_bvals);
-  f(op->d_sweeps);
+  f(op->d_sweeps_blk);
   f(op->d_count);
 }
 
@@ -471,7 +471,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
     }
   if ((rc = upload(&op->d_hxy, hxy)) || (rc = upload(&op->d_nbr, nbr)) ||
       (rc = upload(&op->d_fid, fid)) || (rc = upload(&op->d_finfo, finfo)) ||
-      (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps, 1)) ||
+      (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps_blk, (size_t)NB)) ||
       (rc = dalloc(&op->d_count, 1)))
     {
       free_all(op);
@@ -932,10 +932,14 @@ int oc_dg_last_sweeps(oc_dg *op, int64_t *sweeps, void *stream)
   if (!op || !sweeps)
     return OC_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
-  unsigned long long h = 0;
-  HIP_TRY(hipMemcpyAsync(&h, op->d_sweeps, sizeof(h), hipMemcpyDeviceToHost, s));
+  std::vector<unsigned int> h(op->NB);
+  HIP_TRY(hipMemcpyAsync(h.data(), op->d_sweeps_blk, h.size() * sizeof(unsigned int),
+                         hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  *sweeps = (int64_t)h;
+  int64_t t = 0;
+  for (unsigned int v : h)
+    t += v;
+  *sweeps = t;
   return OC_OK;
 }
 

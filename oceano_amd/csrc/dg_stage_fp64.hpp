@@ -69,10 +69,17 @@ __device__ __forceinline__ bool boundary_state(const Phys<T> &M, int kind, T zm,
 
 // Face terms of one cell: for every face point, `inner` / `bdry` return NC values that are
 // multiplied by JxW and integrated against this cell's face basis into R[NC][nd].
-template <int P, int NC, typename InnerFn, typename BdryFn>
+// own-cell data accessors: registers (stride 1) or the lane-interleaved LDS copy
+struct LdsVec
+{
+  const double *p;  // &lds[component base + lane]
+  __device__ __forceinline__ double operator[](int k) const { return p[k * CB]; }
+};
+
+template <int P, int NC, typename ZA, typename QA, typename InnerFn, typename BdryFn>
 __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &T, int batch,
-                                           int lane, double hx, double hy, const double *Z,
-                                           const double *QX, const double *QY,
+                                           int lane, double hx, double hy, const ZA &Z,
+                                           const QA &QX, const QA &QY,
                                            const double *__restrict__ src_h,
                                            const double *__restrict__ src_q,
                                            double (&R)[NC][(P + 1) * (P + 1)], InnerFn inner,
@@ -82,6 +89,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     {
+      phase_barrier();
       const size_t fo = ((size_t)batch * 4 + f);
       const int info = O.finfo[fo * CB + lane];
       const int kind = info & 3;
@@ -101,7 +109,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
           const int bk = O.bkind[bf];
           const Vec2<double> n = {f == 0 ? -1.0 : f == 1 ? 1.0 : 0.0,
                                   f == 2 ? -1.0 : f == 3 ? 1.0 : 0.0};
-#pragma unroll
+#pragma unroll 1
           for (int a = 0; a < NQ; ++a)
             {
               double zo = 0, xo = 0, yo = 0;
@@ -136,6 +144,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
             {
               if (s == 1 && kind != FK_COARSE)
                 break;
+              phase_barrier();
               const int c2 = O.nbr[(fo * 2 + s) * CB + lane];
               const int fi = O.fid[(fo * 2 + s) * CB + lane];
               const int b2 = c2 / CB, l2 = c2 % CB;
@@ -149,7 +158,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
                   yn_l[k] = src_q[vix(b2, 2, 1, ND, d, l2)];
                 }
               const double len = (kind == FK_COARSE) ? 0.5 * flen : flen;
-#pragma unroll
+#pragma unroll 1
               for (int a = 0; a < NG; ++a)
                 {
                   double mo[N1];
@@ -240,35 +249,23 @@ __device__ __forceinline__ void tensor_inverse(const DegreeTables &T, double *U,
 }
 
 // ---------------------------------------------------------------------------------------
-// L_zeta (INVERT: M^-1 L_zeta, for apply())
+// L_zeta of one cell (cell term + faces) into R; Z, QX, QY: the cell's own data
 // ---------------------------------------------------------------------------------------
-template <int P, bool INVERT>
-__global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__restrict__ src_h,
-                                                   const double *__restrict__ src_q,
-                                                   double *__restrict__ dst)
+template <int P, typename ZA, typename QA>
+__device__ __forceinline__ void height_operator(const OpArgs &O, const DegreeTables &T,
+                                                const Phys<double> &M, int batch, int lane,
+                                                double hx, double hy, const ZA &Z, const QA &QX,
+                                                const QA &QY, const double *__restrict__ src_h,
+                                                const double *__restrict__ src_q,
+                                                double (&R)[1][(P + 1) * (P + 1)])
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
-  const DegreeTables &T = c_tab[P - 1];
-  const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
-  const int lane = threadIdx.x;
-  if (batch * CB + lane >= O.n_cells)
-    return;
-  const Phys<double> M{O.g, O.cu4, O.nu};
-  const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
-  const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
   const double ihx = 1.0 / hx, ihy = 1.0 / hy, area = hx * hy;
-  double Z[ND], QX[ND], QY[ND];
-  double R[1][ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k)
-    {
-      Z[k] = src_h[vix(batch, 1, 0, ND, k, lane)];
-      QX[k] = src_q[vix(batch, 2, 0, ND, k, lane)];
-      QY[k] = src_q[vix(batch, 2, 1, ND, k, lane)];
-      R[0][k] = 0.0;
-    }
+    R[0][k] = 0.0;
   // cell term: integral of grad(phi) . q  (mass_flux = q, ShallowWater.h:352-357)
-#pragma unroll
+#pragma unroll 1
   for (int b = 0; b < NQ; ++b)
     {
       double xs[N1], ys[N1], WS[N1], WD[N1];
@@ -312,6 +309,7 @@ __global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__res
         for (int i = 0; i < N1; ++i)
           R[0][i + N1 * j] += T.Sq[b][j] * WS[i] + T.Dq[b][j] * WD[i];
     }
+  phase_barrier();
   face_terms<P, 1>(
     O, T, batch, lane, hx, hy, Z, QX, QY, src_h, src_q, R,
     [&](bool is_m, double zo, Vec2<double> qo, double zn, Vec2<double> qn, Vec2<double> n,
@@ -328,20 +326,15 @@ __global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__res
       boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
       val[0] = -M.mass_flux(zo, zp, qo, qp, n, zb, zb);
     });
-  if (INVERT)
-    tensor_inverse<P>(T, R[0], 1.0 / area);
-#pragma unroll
-  for (int k = 0; k < ND; ++k)
-    dst[vix(batch, 1, 0, ND, k, lane)] = R[0][k];
 }
 
-// ---------------------------------------------------------------------------------------
-// L_q; MODE 0: stage (M^-1, RK update), 1: apply (M^-1 -> dst), 2: rhs (L_q -> dst)
-// ---------------------------------------------------------------------------------------
-template <int P, bool VISC, int MODE>
-__global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
+// L_zeta (INVERT: M^-1 L_zeta, for apply())
+template <int P, bool INVERT>
+__global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__restrict__ src_h,
+                                                   const double *__restrict__ src_q,
+                                                   double *__restrict__ dst)
 {
-  constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
+  constexpr int ND = (P + 1) * (P + 1);
   const DegreeTables &T = c_tab[P - 1];
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
   const int lane = threadIdx.x;
@@ -350,27 +343,81 @@ __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
   const Phys<double> M{O.g, O.cu4, O.nu};
   const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
   const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
-  const double ihx = 1.0 / hx, ihy = 1.0 / hy, area = hx * hy;
-  const double *__restrict__ src_h = S.src_h;
-  const double *__restrict__ src_q = S.src_q;
-  double Z[ND], QX[ND], QY[ND], UX[VISC ? ND : 1], UY[VISC ? ND : 1];
-  double R[2][ND];
+  double Z[ND], QX[ND], QY[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k)
     {
       Z[k] = src_h[vix(batch, 1, 0, ND, k, lane)];
       QX[k] = src_q[vix(batch, 2, 0, ND, k, lane)];
       QY[k] = src_q[vix(batch, 2, 1, ND, k, lane)];
-      if (VISC)
-        {
-          UX[k] = S.src_v[vix(batch, 2, 0, ND, k, lane)];
-          UY[k] = S.src_v[vix(batch, 2, 1, ND, k, lane)];
-        }
+    }
+  double R[1][ND];
+  height_operator<P>(O, T, M, batch, lane, hx, hy, (const double *)Z, (const double *)QX,
+                     (const double *)QY, src_h, src_q, R);
+  if (INVERT)
+    tensor_inverse<P>(T, R[0], 1.0 / (hx * hy));
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+    dst[vix(batch, 1, 0, ND, k, lane)] = R[0][k];
+}
+
+// ---------------------------------------------------------------------------------------
+// L_q of one cell; MODE 0: stage (M^-1, RK update), 1: apply (M^-1 -> dst), 2: rhs
+// The cell's own zeta, q (and velocity) are staged in LDS, lane-interleaved.
+// ---------------------------------------------------------------------------------------
+template <int P, bool VISC>
+constexpr int discharge_lds_doubles()
+{
+  return (VISC ? 5 : 3) * (P + 1) * (P + 1) * CB;
+}
+
+template <int P, bool VISC, int MODE>
+__device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs &S, int batch,
+                                                double *lds)
+{
+  constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
+  const DegreeTables &T = c_tab[P - 1];
+  const int lane = threadIdx.x;
+  const double *__restrict__ src_h = S.src_h;
+  const double *__restrict__ src_q = S.src_q;
+  // stage the batch's own data: contiguous, coalesced
+  {
+    const double *gh = src_h + (size_t)batch * ND * CB;
+    const double *gq = src_q + (size_t)batch * 2 * ND * CB;
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+      {
+        lds[k * CB + lane] = gh[k * CB + lane];
+        lds[(ND + k) * CB + lane] = gq[k * CB + lane];
+        lds[(2 * ND + k) * CB + lane] = gq[(ND + k) * CB + lane];
+        if (VISC)
+          {
+            const double *gv = S.src_v + (size_t)batch * 2 * ND * CB;
+            lds[(3 * ND + k) * CB + lane] = gv[k * CB + lane];
+            lds[(4 * ND + k) * CB + lane] = gv[(ND + k) * CB + lane];
+          }
+      }
+  }
+  // each lane reads only its own column of the staged data: no barrier needed beyond the
+  // wave's own ordering of its LDS writes and reads
+  __builtin_amdgcn_wave_barrier();
+  if (batch * CB + lane >= O.n_cells)
+    return;
+  const LdsVec Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
+  const LdsVec UX{lds + 3 * ND * CB + lane}, UY{lds + 4 * ND * CB + lane};
+  const Phys<double> M{O.g, O.cu4, O.nu};
+  const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
+  const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
+  const double ihx = 1.0 / hx, ihy = 1.0 / hy, area = hx * hy;
+  double R[2][ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+    {
       R[0][k] = 0.0;
       R[1][k] = 0.0;
     }
   const size_t c0base = (size_t)batch * NQ * NQ * CB + lane;
-#pragma unroll
+#pragma unroll 1
   for (int b = 0; b < NQ; ++b)
     {
       double zs[N1], zd[N1], xs[N1], ys[N1];
@@ -383,8 +430,9 @@ __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
 #pragma unroll
           for (int j = 0; j < N1; ++j)
             {
-              s0 += T.Sq[b][j] * Z[i + N1 * j];
-              s1 += T.Dq[b][j] * Z[i + N1 * j];
+              const double zv = Z[i + N1 * j];
+              s0 += T.Sq[b][j] * zv;
+              s1 += T.Dq[b][j] * zv;
               s2 += T.Sq[b][j] * QX[i + N1 * j];
               s3 += T.Sq[b][j] * QY[i + N1 * j];
             }
@@ -431,7 +479,6 @@ __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
           const double h = M.depth(z, zb);
           const double s = M.inv_depth(h);
           const double vx = qx * s, vy = qy * s;
-          // advective (+ diffusive) flux F[e][d]
           double F00 = qx * vx, F01 = qx * vy, F10 = qy * vx, F11 = qy * vy;
           if (VISC)
             {
@@ -454,7 +501,6 @@ __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
               F10 = F10 - O.nu * (G10 + G01);
               F11 = F11 - O.nu * (G11 + G11) + O.nu * div;
             }
-          // source: -g h grad(zeta) - friction + wind + coriolis
           const double mgh = -O.g * h;
           double Sx = mgh * zx, Sy = mgh * zy;
           if (O.has_friction)
@@ -498,6 +544,7 @@ __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
             R[1][i + N1 * j] += T.Sq[b][j] * WSY[i] + T.Dq[b][j] * WDY[i];
           }
     }
+  phase_barrier();
   face_terms<P, 2>(
     O, T, batch, lane, hx, hy, Z, QX, QY, src_h, src_q, R,
     [&](bool is_m, double zo, Vec2<double> qo, double zn, Vec2<double> qn, Vec2<double> n,
@@ -548,6 +595,7 @@ __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
         for (int k = 0; k < ND; ++k)
           S.scratch[vix(batch, 2, c, ND, k, lane)] = R[c][k];
     }
+  phase_barrier();
   const double inv_area = 1.0 / area;
   tensor_inverse<P>(T, R[0], inv_area);
   tensor_inverse<P>(T, R[1], inv_area);
@@ -573,6 +621,14 @@ __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
     }
 }
 
+template <int P, bool VISC, int MODE>
+__global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
+{
+  __shared__ double lds[discharge_lds_doubles<P, VISC>()];
+  discharge_block<P, VISC, MODE>(O, S, logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle),
+                                 lds);
+}
+
 // ---------------------------------------------------------------------------------------
 // front + capped lumped-Jacobi depth-mass solve
 // ---------------------------------------------------------------------------------------
@@ -586,7 +642,7 @@ __device__ __forceinline__ int depth_mass(const OpArgs &O, const DegreeTables &T
   constexpr int N1 = P + 1, NG = P + 2;
   const size_t zbase = (size_t)batch * NG * NG * CB + lane;
   int ndry = 0;
-#pragma unroll
+#pragma unroll 1
   for (int b = 0; b < NG; ++b)
     {
       double zs[N1], W[N1];
@@ -628,10 +684,130 @@ __device__ __forceinline__ int depth_mass(const OpArgs &O, const DegreeTables &T
   return ndry;
 }
 
+// Jacobi solve of one cell: Z holds zeta^n on entry (and the dry lift is applied to it),
+// F the front; returns the number of sweeps.  OceanDG.h:1374-1507.
+template <int P>
+__device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables &T,
+                                            const Phys<double> &M, int batch, int lane,
+                                            double area, const double *F, double *Z)
+{
+  constexpr int N1 = P + 1, ND = N1 * N1, NG = P + 2, NP = NG * NG;
+  // dry-cell lift (OceanDG.h:1399-1422)
+  {
+    const size_t zbase = (size_t)batch * NP * CB + lane;
+    double zbmax = -DBL_MAX;
+    int ndry = 0;
+#pragma unroll 1
+    for (int b = 0; b < NG; ++b)
+      {
+        double zs[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i)
+          {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < N1; ++j)
+              s += T.Sg[b][j] * Z[i + N1 * j];
+            zs[i] = s;
+          }
+#pragma unroll
+        for (int a = 0; a < NG; ++a)
+          {
+            double z = 0;
+#pragma unroll
+            for (int i = 0; i < N1; ++i)
+              z += T.Sg[a][i] * zs[i];
+            const double zb = fld(O.cell1, zbase + (size_t)(a + NG * b) * CB);
+            if (z <= -zb)
+              {
+                zbmax = zb > zbmax ? zb : zbmax;
+                ++ndry;
+              }
+          }
+      }
+    if (ndry == NP)
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        Z[i] = -zbmax + 1e-09;
+  }
+  double norm = 1.0;
+  int k = 0;
+  for (; k < O.max_it && norm > 1e-12; ++k)
+    {
+      double Rr[ND];
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        Rr[i] = F[i];
+      unsigned long long bits = 0;
+      const int ndry = depth_mass<P, true>(O, T, M, batch, lane, Z, Rr, -1.0, area, bits);
+      double D[ND];
+      if (ndry == 0)
+        {
+#pragma unroll
+          for (int j = 0; j < N1; ++j)
+#pragma unroll
+            for (int i = 0; i < N1; ++i)
+              D[i + N1 * j] = T.lumped[i] * T.lumped[j] * area;
+        }
+      else
+        {
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            D[i] = 0;
+#pragma unroll 1
+          for (int b = 0; b < NG; ++b)
+            {
+              double W[N1];
+#pragma unroll
+              for (int i = 0; i < N1; ++i)
+                W[i] = 0;
+#pragma unroll
+              for (int a = 0; a < NG; ++a)
+                {
+                  const double mask = ((bits >> (a + NG * b)) & 1ull) ? 0.0 : 1.0;
+                  const double v = mask * (T.wg[a] * T.wg[b] * area);
+#pragma unroll
+                  for (int i = 0; i < N1; ++i)
+                    W[i] += T.Sg[a][i] * v;
+                }
+#pragma unroll
+              for (int j = 0; j < N1; ++j)
+#pragma unroll
+                for (int i = 0; i < N1; ++i)
+                  D[i + N1 * j] += T.Sg[b][j] * W[i];
+            }
+        }
+      norm = 0.0;
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        {
+          const double dz = (ndry < NP) ? (1. / D[i]) * Rr[i] : 0.0;
+          const double old = Z[i];
+          Z[i] += dz;
+          norm += fabs(old - Z[i]);
+        }
+    }
+  return k;
+}
+
+__device__ __forceinline__ void store_sweeps(unsigned int *blk, int batch, int k)
+{
+  if (!blk)
+    return;
+  unsigned int v = (unsigned int)k;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off);
+  if (threadIdx.x == 0)
+    blk[batch] = v;
+}
+
+// front from ki_h[1..stage+1] (all in memory) and the Jacobi solve: the path used when
+// the stage's outputs alias its inputs
 template <int P>
 __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
 {
-  constexpr int N1 = P + 1, ND = N1 * N1, NG = P + 2, NP = NG * NG;
+  constexpr int N1 = P + 1, ND = N1 * N1;
   const DegreeTables &T = c_tab[P - 1];
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
   const int lane = threadIdx.x;
@@ -660,113 +836,87 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
 #pragma unroll
         for (int i = 0; i < ND; ++i)
           S.scratch[vix(batch, 1, 0, ND, i, lane)] = F[i];
-      // dry-cell lift (OceanDG.h:1399-1422)
-      {
-        const size_t zbase = (size_t)batch * NP * CB + lane;
-        double zbmax = -DBL_MAX;
-        int ndry = 0;
-#pragma unroll
-        for (int b = 0; b < NG; ++b)
-          {
-            double zs[N1];
-#pragma unroll
-            for (int i = 0; i < N1; ++i)
-              {
-                double s = 0;
-#pragma unroll
-                for (int j = 0; j < N1; ++j)
-                  s += T.Sg[b][j] * Z[i + N1 * j];
-                zs[i] = s;
-              }
-#pragma unroll
-            for (int a = 0; a < NG; ++a)
-              {
-                double z = 0;
-#pragma unroll
-                for (int i = 0; i < N1; ++i)
-                  z += T.Sg[a][i] * zs[i];
-                const double zb = fld(O.cell1, zbase + (size_t)(a + NG * b) * CB);
-                if (z <= -zb)
-                  {
-                    zbmax = zb > zbmax ? zb : zbmax;
-                    ++ndry;
-                  }
-              }
-          }
-        if (ndry == NP)
-#pragma unroll
-          for (int i = 0; i < ND; ++i)
-            Z[i] = -zbmax + 1e-09;
-      }
-      double norm = 1.0;
-      for (; k < O.max_it && norm > 1e-12; ++k)
-        {
-          double Rr[ND];
-#pragma unroll
-          for (int i = 0; i < ND; ++i)
-            Rr[i] = F[i];
-          unsigned long long bits = 0;
-          const int ndry = depth_mass<P, true>(O, T, M, batch, lane, Z, Rr, -1.0, area, bits);
-          double D[ND];
-          if (ndry == 0)
-            {
-#pragma unroll
-              for (int j = 0; j < N1; ++j)
-#pragma unroll
-                for (int i = 0; i < N1; ++i)
-                  D[i + N1 * j] = T.lumped[i] * T.lumped[j] * area;
-            }
-          else
-            {
-#pragma unroll
-              for (int i = 0; i < ND; ++i)
-                D[i] = 0;
-#pragma unroll
-              for (int b = 0; b < NG; ++b)
-                {
-                  double W[N1];
-#pragma unroll
-                  for (int i = 0; i < N1; ++i)
-                    W[i] = 0;
-#pragma unroll
-                  for (int a = 0; a < NG; ++a)
-                    {
-                      const double mask = ((bits >> (a + NG * b)) & 1ull) ? 0.0 : 1.0;
-                      const double v = mask * (T.wg[a] * T.wg[b] * area);
-#pragma unroll
-                      for (int i = 0; i < N1; ++i)
-                        W[i] += T.Sg[a][i] * v;
-                    }
-#pragma unroll
-                  for (int j = 0; j < N1; ++j)
-#pragma unroll
-                    for (int i = 0; i < N1; ++i)
-                      D[i + N1 * j] += T.Sg[b][j] * W[i];
-                }
-            }
-          norm = 0.0;
-#pragma unroll
-          for (int i = 0; i < ND; ++i)
-            {
-              const double dz = (ndry < NP) ? (1. / D[i]) * Rr[i] : 0.0;
-              const double old = Z[i];
-              Z[i] += dz;
-              norm += fabs(old - Z[i]);
-            }
-        }
+      k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         S.rk_dst[vix(batch, 1, 0, ND, i, lane)] = Z[i];
     }
-  if (S.sweeps)
+  (void)N1;
+  store_sweeps(S.sweeps_blk, batch, k);
+}
+
+// L_zeta, front and Jacobi solve of one batch of cells in one pass: L_zeta stays in
+// registers for the front (it is also written out as ki_h[stage+1] for later stages)
+template <int P>
+__device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageArgs &S,
+                                                   int batch)
+{
+  constexpr int ND = (P + 1) * (P + 1);
+  const DegreeTables &T = c_tab[P - 1];
+  const int lane = threadIdx.x;
+  const bool active = batch * CB + lane < O.n_cells;
+  const Phys<double> M{O.g, O.cu4, O.nu};
+  int k = 0;
+  if (active)
     {
-      unsigned long long v = (unsigned long long)k;
+      const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
+      const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
+      const double area = hx * hy;
+      double F[ND];
+      {
+        double Z[ND], QX[ND], QY[ND];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off);
-      if (lane == 0)
-        atomicAdd(S.sweeps, v);
+        for (int i = 0; i < ND; ++i)
+          {
+            Z[i] = S.src_h[vix(batch, 1, 0, ND, i, lane)];
+            QX[i] = S.src_q[vix(batch, 2, 0, ND, i, lane)];
+            QY[i] = S.src_q[vix(batch, 2, 1, ND, i, lane)];
+          }
+        double R[1][ND];
+        height_operator<P>(O, T, M, batch, lane, hx, hy, (const double *)Z,
+                           (const double *)QX, (const double *)QY, S.src_h, S.src_q, R);
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          {
+            const size_t idx = vix(batch, 1, 0, ND, i, lane);
+            S.dst[idx] = R[0][i];  // ki_h[stage+1]
+            const double kk = R[0][i];
+            double f = S.factor[0] * (S.stage == 0 ? kk : S.kprev[0][idx]);
+            for (int j = 1; j <= S.stage; ++j)
+              f += S.factor[j] * (j == S.stage ? kk : S.kprev[j][idx]);
+            F[i] = f;
+          }
+      }
+      phase_barrier();
+      double Z[ND];
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        Z[i] = S.wsol[vix(batch, 1, 0, ND, i, lane)];
+      unsigned long long bits = 0;
+      depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
+      if (S.scratch)
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          S.scratch[vix(batch, 1, 0, ND, i, lane)] = F[i];
+      k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        S.rk_dst[vix(batch, 1, 0, ND, i, lane)] = Z[i];
     }
+  store_sweeps(S.sweeps_blk, batch, k);
+}
+
+// the whole stage in one launch: work items w = (batch, kind) are dealt so that each XCD
+// holds a contiguous range of batches; kind 0 = height stage, kind 1 = discharge stage
+template <int P, bool VISC>
+__global__ __launch_bounds__(CB) void k_stage(OpArgs O, StageArgs H, StageArgs Q)
+{
+  __shared__ double lds[discharge_lds_doubles<P, VISC>()];
+  const int w = logical_batch(blockIdx.x, 2 * O.n_batches, O.xcd_swizzle);
+  if (w & 1)
+    discharge_block<P, VISC, 0>(O, Q, w >> 1, lds);
+  else
+    height_stage_block<P>(O, H, w >> 1);
 }
 
 }  // namespace oceano

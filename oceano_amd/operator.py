@@ -190,15 +190,15 @@ class OceanoOperator:
               "oc_dg_count_nonfinite")
         return int(n[0])
 
-    KERNELS = ("height_rhs", "discharge", "height_jacobi")
+    KERNELS = ("stage", "height_rhs", "discharge", "rk_update", "height_jacobi")
 
     def set_timing(self, enable):
         check(lib().oc_dg_set_timing(self._h, int(enable)), "oc_dg_set_timing")
 
     def timing_report(self):
         """{kernel: (total ms, launches)} since the last report (events on the stream)."""
-        ms = np.zeros(3)
-        cnt = np.zeros(3, np.int64)
+        ms = np.zeros(5)
+        cnt = np.zeros(5, np.int64)
         check(lib().oc_dg_timing_report(self._h, dptr(ms), cnt.ctypes.data_as(_capi.PI64)),
               "oc_dg_timing_report")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.KERNELS)}
