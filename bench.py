@@ -15,6 +15,8 @@ bounded sample of the same workload.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -85,7 +87,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed RK stages")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--dt", type=float, default=1e-5)
+    ap.add_argument("--dt", type=float, default=2e-5,
+                    help="RK4 step (CFL ~0.03 on the finest cells; 6.5e-5 is unstable)")
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -99,50 +103,65 @@ def main():
     from oceano_amd.integrator import butcher_rows
 
     P = build_problem()
-    op, data, w_h, w_q = problems.setup_operator(P)
     rows, _ = butcher_rows("stage_4_order_4", args.dt)
     S = len(rows)
-    ki_h = [op.initialize_vector(0) for _ in range(S + 1)]
-    ki_q = [op.initialize_vector(1) for _ in range(S + 1)]
-    r_h = [op.initialize_vector(0) for _ in range(2)]
-    r_q = [op.initialize_vector(1) for _ in range(2)]
 
-    def stage(k):
-        s = k % S
-        ri = [w_h, w_q] if s == 0 else [r_h[s % 2], r_q[s % 2]]
-        op.perform_stage_hydro(s, 0.0, rows[s][:s + 1], ri + [None, None], ki_h, ki_q, w_h,
-                               w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
+    def run(precision):
+        op, data, w_h, w_q = problems.setup_operator(P, precision=precision)
+        ki_h = [op.initialize_vector(0) for _ in range(S + 1)]
+        ki_q = [op.initialize_vector(1) for _ in range(S + 1)]
+        r_h = [op.initialize_vector(0) for _ in range(2)]
+        r_q = [op.initialize_vector(1) for _ in range(2)]
 
-    for k in range(args.warmup):
-        stage(k)
-    torch.cuda.synchronize()
-    op.set_timing(True)
-    k0 = args.warmup - args.warmup % S  # start the timed region on stage 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        stage(k0 + k)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    rep = op.timing_report()
-    op.set_timing(False)
-    sweeps = op.last_sweeps()
-    nonfinite = op.count_nonfinite(w_h) + op.count_nonfinite(w_q)
+        def stage(k):
+            s = k % S
+            ri = [w_h, w_q] if s == 0 else [r_h[s % 2], r_q[s % 2]]
+            op.perform_stage_hydro(s, 0.0, rows[s][:s + 1], ri + [None, None], ki_h, ki_q,
+                                   w_h, w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
 
+        k0 = args.warmup + (-args.warmup) % S  # warm up whole steps: timing starts at stage 0
+        for k in range(k0):
+            stage(k)
+        torch.cuda.synchronize()
+        op.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            stage(k0 + k)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        rep = op.timing_report()
+        op.set_timing(False)
+        # complete the last time step so the state is at a step boundary
+        k = k0 + args.steps
+        while k % S:
+            stage(k)
+            k += 1
+        torch.cuda.synchronize()
+        n_steps = k // S
+        err = op.compute_errors(w_h, w_q, P.exact, n_steps * args.dt)
+        static = {}  # vortex: all problem-data fields uniform, none stored
+        per_stage = [algorithmic_bytes(op, s, static) for s in range(S)]
+        kern = {}
+        for name, (ms, n) in rep.items():
+            if n == 0:
+                continue
+            b = sum(per_stage[(k0 + j) % S][name] for j in range(args.steps)) / args.steps
+            mean_ms = ms / n
+            kern[name] = {"mean_ms": mean_ms, "launches": n, "alg_bytes": b,
+                          "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
+        return {"op": op, "elapsed": elapsed, "kernels": kern, "errors": err,
+                "state": (op.unpack(w_h, 1).cpu().numpy(), op.unpack(w_q, 2).cpu().numpy()),
+                "nonfinite": op.count_nonfinite(w_h) + op.count_nonfinite(w_q),
+                "sweeps_last_stage": op.last_sweeps(), "t_end": n_steps * args.dt}
+
+    precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {p: run(p) for p in precisions}
+    head = runs["mixed"] if "mixed" in runs else runs["fp64"]
     dofs = P.dofs()
-    value = dofs * args.steps / elapsed
-    static = {}  # vortex: all problem-data fields uniform, none stored
-    per_stage = [algorithmic_bytes(op, s, static) for s in range(S)]
-    kern = {}
-    for name, (ms, n) in rep.items():
-        if n == 0:
-            continue
-        b = sum(per_stage[(k0 + k) % S][name] for k in range(args.steps)) / args.steps
-        mean_ms = ms / max(n, 1)
-        kern[name] = {"mean_ms": mean_ms, "launches": n, "alg_bytes": b,
-                      "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
-    dom = max(kern, key=lambda k: kern[k]["mean_ms"])
-    achieved = kern[dom]["alg_GBs"]
+    value = dofs * args.steps / head["elapsed"]
+    dom = max(head["kernels"], key=lambda k: head["kernels"][k]["mean_ms"])
+    achieved = head["kernels"][dom]["alg_GBs"]
     line = {
         "metric": "DoF-RK-stages/s on 1 GPU, mixed vs FP64 at matched L2 error; % HBM peak",
         "value": value,
@@ -150,24 +169,38 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": head["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if "mixed" in runs else "f64",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "cells": P.mesh.n_cells,
                    "hanging_subfaces": P.mesh.n_hanging, "dofs": dofs, "degree": 3,
                    "scheme": "classic RK4 (Butcher), stage = step", "dt": args.dt,
-                   "precision": "fp64"},
+                   "precision": "mixed" if "mixed" in runs else "fp64"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
                      "traffic": None},
-        "kernels": kern,
-        "jacobi_sweeps_last_stage": sweeps,
-        "nonfinite": nonfinite,
     }
+    for p, r in runs.items():
+        line[p] = {"value": dofs * args.steps / r["elapsed"],
+                   "ms_per_step": r["elapsed"] / args.steps * 1e3, "kernels": r["kernels"],
+                   "l2_error_vs_exact": {k: float(v) for k, v in r["errors"].items()},
+                   "t_end": r["t_end"], "nonfinite": r["nonfinite"],
+                   "jacobi_sweeps_last_stage": r["sweeps_last_stage"]}
+    if len(runs) == 2:
+        (fh, fq), (mh, mq) = runs["fp64"]["state"], runs["mixed"]["state"]
+        nd = runs["fp64"]["op"].dofs_per_cell
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        line["mixed_vs_fp64"] = {
+            "speedup": line["mixed"]["value"] / line["fp64"]["value"],
+            "rel_l2_zeta": rel(mh, fh),
+            "rel_l2_hu": rel(mq.reshape(-1, 2, nd)[:, 0], fq.reshape(-1, 2, nd)[:, 0]),
+            "rel_l2_hv": rel(mq.reshape(-1, 2, nd)[:, 1], fq.reshape(-1, 2, nd)[:, 1])}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, args.cpu_stages)
     else:

@@ -186,6 +186,15 @@ int oc_dg_rhs(oc_dg *op, const double *src_h, const double *src_q, const double 
 int oc_dg_project_collocation(const oc_dg *op, int32_t n_comp, const double *values,
                               double *coeffs);
 
+/* Errors against an exact solution: compute_errors_hydro (OceanDG.h:2382-2459) with the
+ * fields reported separately.  zeta [n_cells][n1^2] and q [n_cells][2][n1^2] cell-major,
+ * exact [n_cells][nq^2][3] = (free surface, u, v) at the OC_STORE_CELL_GAUSS points.
+ * err[0] = L2 of the depth error, err[1] = L2 of the discharge error (hu, hv together,
+ * as the reference), err[2], err[3] = L2 of hu, hv, err[4..6] = Linf of depth, hu, hv.
+ * Depths use the z_b of OC_STORE_CELL_GAUSS.  Host memory, no GPU work. */
+int oc_dg_compute_errors(const oc_dg *op, const double *zeta, const double *q,
+                         const double *exact, double err[7]);
+
 /* Count non-finite entries of a device vector of `n` doubles into *count (host).  The
  * reference has no such check (main.cc:1527-1529 only mentions blow-up); hosts use it
  * to turn a NaN state into OC_ERR_NONFINITE. Synchronises the stream. */

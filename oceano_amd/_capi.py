@@ -85,6 +85,7 @@ SIGNATURES = {
     "oc_dg_count_nonfinite": (C.c_int, [P, P, I64, PI64, P]),
     "oc_dg_last_sweeps": (C.c_int, [P, PI64, P]),
     "oc_dg_project_collocation": (C.c_int, [P, I32, PD, PD]),
+    "oc_dg_compute_errors": (C.c_int, [P, PD, PD, PD, PD]),
     "oc_dg_set_timing": (C.c_int, [P, I32]),
     "oc_dg_timing_report": (C.c_int, [P, PD, PI64]),
     "oc_icbc_vortex": (C.c_int, [PD, D, I64, PD, PD]),

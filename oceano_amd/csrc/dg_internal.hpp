@@ -42,8 +42,11 @@ struct oc_dg
   double c_bz = 0;
   double *d_bvals = nullptr;
   unsigned int *d_sweeps_blk = nullptr;
+  float *d_jmat = nullptr;  // mixed precision: closed-form Jacobi matrix (unit cell)
+  double wet_margin = 10.0;
   unsigned long long *d_count = nullptr;
   std::vector<double> cell2_host;
+  std::vector<double> zb_gauss_host;  // z_b at the OC_STORE_CELL_GAUSS points
   int xcd_swizzle = 1;
   // optional per-kernel event timing: kernel id -> recorded (start, stop) pairs
   int timing = 0;
@@ -75,6 +78,8 @@ struct oc_dg
     o.has_coriolis = (d_cell0[4] != nullptr) || (c_cell0[4] != 0.0);
     o.max_it = max_it;
     o.xcd_swizzle = xcd_swizzle;
+    o.jmat = d_jmat;
+    o.wet_margin = wet_margin;
     return o;
   }
 };
@@ -96,7 +101,7 @@ namespace oceano {
 
 // per-degree translation units (dg_stage_p<P>.hip)
 #define OC_DECLARE_DEGREE(P)                                                                \
-  int upload_tables_d##P(const DegreeTables *t);                                            \
+  int upload_tables_d##P(const DegreeTables *t, const Basis<float> *tf);                  \
   int stage_d##P(oc_dg *op, const OpArgs &O, uint32_t stage, const double *factor,          \
                  const double *ri_h, const double *ri_q, const double *ri_v,                \
                  double *const *ki_h, double *const *ki_q, uint32_t n_ki, double *w_h,      \

@@ -21,18 +21,20 @@ namespace oceano {
 constexpr int CB = OC_CELL_BATCH;
 constexpr int MAXN1 = 5, MAXNQ = 7, MAXNG = 6, MAXKI = 9;
 
-struct DegreeTables
+template <typename R>
+struct Basis
 {
-  double Sq[MAXNQ][MAXN1];      // l_i at Gauss(nq) points
-  double Dq[MAXNQ][MAXN1];      // l_i' at Gauss(nq) points
-  double wq[MAXNQ];
-  double Sg[MAXNG][MAXN1];      // l_i at Gauss-Lobatto(ng) points
-  double Sgs[2][MAXNG][MAXN1];  // l_i at (t + s)/2, t in GL(ng): coarse side of subface s
-  double wg[MAXNG];
-  double Ti[MAXN1][MAXN1];      // inverse of [l_i(Gauss(n1)_a)]_{a,i}, stored [i][a]
-  double wn[MAXN1];             // Gauss(n1) weights
-  double lumped[MAXN1];         // sum_a Sg[a][i] wg[a]
+  R Sq[MAXNQ][MAXN1];      // l_i at Gauss(nq) points
+  R Dq[MAXNQ][MAXN1];      // l_i' at Gauss(nq) points
+  R wq[MAXNQ];
+  R Sg[MAXNG][MAXN1];      // l_i at Gauss-Lobatto(ng) points
+  R Sgs[2][MAXNG][MAXN1];  // l_i at (t + s)/2, t in GL(ng): coarse side of subface s
+  R wg[MAXNG];
+  R Ti[MAXN1][MAXN1];      // inverse of [l_i(Gauss(n1)_a)]_{a,i}, stored [i][a]
+  R wn[MAXN1];             // Gauss(n1) weights
+  R lumped[MAXN1];         // sum_a Sg[a][i] wg[a]
 };
+using DegreeTables = Basis<double>;
 
 enum FaceKind
 {
@@ -66,6 +68,10 @@ struct OpArgs
   int has_friction, has_wind, has_coriolis;
   int max_it;
   int xcd_swizzle;
+  // mixed precision: closed form of max_it linear Jacobi sweeps on a wet unit cell,
+  // [nd][nd] row-major (delta = J r / area), and the wet-cell safety factor
+  const float *jmat;
+  double wet_margin;
 };
 
 struct StageArgs

@@ -2,7 +2,7 @@
 #pragma once
 
 #include "dg_internal.hpp"
-#include "dg_stage_fp64.hpp"
+#include "dg_stage.hpp"
 
 namespace oceano {
 namespace {
@@ -31,7 +31,7 @@ __global__ void k_rk_update(RkArgs A)
 }
 
 
-template <int P>
+template <int P, typename R>
 int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *factor,
                  const double *ri_h, const double *ri_q, const double *ri_v,
                  double *const *ki_h, double *const *ki_q, uint32_t n_ki, double *w_h,
@@ -78,24 +78,24 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
     {
       timed_begin(op, s, &ev);
       if (visc)
-        hipLaunchKernelGGL((k_stage<P, true>), dim3(2 * op->NB), block, 0, s, O, H, Q);
+        hipLaunchKernelGGL((k_stage<P, true, R>), dim3(2 * op->NB), block, 0, s, O, H, Q);
       else
-        hipLaunchKernelGGL((k_stage<P, false>), dim3(2 * op->NB), block, 0, s, O, H, Q);
+        hipLaunchKernelGGL((k_stage<P, false, R>), dim3(2 * op->NB), block, 0, s, O, H, Q);
       timed_end(op, s, TK_STAGE, ev);
       HIP_TRY(hipGetLastError());
       return OC_OK;
     }
   // split path: every kernel finishes reading ri before the next one overwrites it
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_height_rhs<P, false>), grid, block, 0, s, O, ri_h, ri_q,
+  hipLaunchKernelGGL((k_height_rhs<P, false, R>), grid, block, 0, s, O, ri_h, ri_q,
                      ki_h[stage + 1]);
   timed_end(op, s, TK_HEIGHT_RHS, ev);
   timed_begin(op, s, &ev);
   StageArgs A = Q;
   if (visc)
-    hipLaunchKernelGGL((k_discharge<P, true, 1>), grid, block, 0, s, O, A);
+    hipLaunchKernelGGL((k_discharge<P, true, 1, R>), grid, block, 0, s, O, A);
   else
-    hipLaunchKernelGGL((k_discharge<P, false, 1>), grid, block, 0, s, O, A);
+    hipLaunchKernelGGL((k_discharge<P, false, 1, R>), grid, block, 0, s, O, A);
   timed_end(op, s, TK_DISCHARGE, ev);
   RkArgs U{};
   U.n = (size_t)op->NB * CB * 2 * op->nd;
@@ -114,22 +114,22 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
   for (uint32_t j = 0; j <= stage; ++j)
     J.kprev[j] = ki_h[j + 1];
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_height_jacobi<P>), grid, block, 0, s, O, J);
+  hipLaunchKernelGGL((k_height_jacobi<P, R>), grid, block, 0, s, O, J);
   timed_end(op, s, TK_HEIGHT_JACOBI, ev);
   HIP_TRY(hipGetLastError());
   return OC_OK;
 }
 
-template <int P>
+template <int P, typename R>
 int launch_rhs_apply(oc_dg *op, const OpArgs &O, bool invert, const double *src_h,
                      const double *src_q, const double *src_v, double *dst_h, double *dst_q,
                      hipStream_t s)
 {
   const dim3 grid(op->NB), block(CB);
   if (invert)
-    hipLaunchKernelGGL((k_height_rhs<P, true>), grid, block, 0, s, O, src_h, src_q, dst_h);
+    hipLaunchKernelGGL((k_height_rhs<P, true, R>), grid, block, 0, s, O, src_h, src_q, dst_h);
   else
-    hipLaunchKernelGGL((k_height_rhs<P, false>), grid, block, 0, s, O, src_h, src_q, dst_h);
+    hipLaunchKernelGGL((k_height_rhs<P, false, R>), grid, block, 0, s, O, src_h, src_q, dst_h);
   StageArgs S{};
   S.src_h = src_h;
   S.src_q = src_q;
@@ -139,16 +139,16 @@ int launch_rhs_apply(oc_dg *op, const OpArgs &O, bool invert, const double *src_
   if (invert)
     {
       if (visc)
-        hipLaunchKernelGGL((k_discharge<P, true, 1>), grid, block, 0, s, O, S);
+        hipLaunchKernelGGL((k_discharge<P, true, 1, R>), grid, block, 0, s, O, S);
       else
-        hipLaunchKernelGGL((k_discharge<P, false, 1>), grid, block, 0, s, O, S);
+        hipLaunchKernelGGL((k_discharge<P, false, 1, R>), grid, block, 0, s, O, S);
     }
   else
     {
       if (visc)
-        hipLaunchKernelGGL((k_discharge<P, true, 2>), grid, block, 0, s, O, S);
+        hipLaunchKernelGGL((k_discharge<P, true, 2, R>), grid, block, 0, s, O, S);
       else
-        hipLaunchKernelGGL((k_discharge<P, false, 2>), grid, block, 0, s, O, S);
+        hipLaunchKernelGGL((k_discharge<P, false, 2, R>), grid, block, 0, s, O, S);
     }
   HIP_TRY(hipGetLastError());
   return OC_OK;

@@ -71,6 +71,60 @@ DegreeTables make_tables(int p)
   return t;
 }
 
+Basis<float> to_float(const DegreeTables &t)
+{
+  Basis<float> f;
+  const double *src = reinterpret_cast<const double *>(&t);
+  float *dst = reinterpret_cast<float *>(&f);
+  static_assert(sizeof(DegreeTables) == 2 * sizeof(Basis<float>), "table layout");
+  for (size_t i = 0; i < sizeof(DegreeTables) / sizeof(double); ++i)
+    dst[i] = float(src[i]);
+  return f;
+}
+
+// K linear Jacobi sweeps on a wet unit cell from a zero increment:
+// J = sum_{k<K} (I - D^-1 M)^k D^-1 with M the GL(p+2) mass (exact) and D its row sums
+std::vector<float> jacobi_matrix(int p, int K)
+{
+  const DegreeTables t = make_tables(p);
+  const int n1 = p + 1, nd = n1 * n1, ng = p + 2;
+  std::vector<double> M1(n1 * n1, 0.0), M(nd * nd), D(nd), A(nd * nd), J(nd * nd, 0.0),
+    P(nd * nd, 0.0);
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j)
+      for (int a = 0; a < ng; ++a)
+        M1[i * n1 + j] += t.Sg[a][i] * t.Sg[a][j] * t.wg[a];
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < nd; ++j)
+      M[i * nd + j] = M1[(i % n1) * n1 + (j % n1)] * M1[(i / n1) * n1 + (j / n1)];
+  for (int i = 0; i < nd; ++i)
+    D[i] = t.lumped[i % n1] * t.lumped[i / n1];
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < nd; ++j)
+      A[i * nd + j] = (i == j ? 1.0 : 0.0) - M[i * nd + j] / D[i];
+  // P = D^-1, then J = sum_k A^k D^-1 by Horner: J = D^-1 + A (D^-1 + A (...))
+  for (int i = 0; i < nd; ++i)
+    P[i * nd + i] = 1.0 / D[i];
+  for (int k = 0; k < K; ++k)
+    {
+      std::vector<double> N(nd * nd, 0.0);
+      for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j)
+          {
+            double s = (i == j) ? 1.0 / D[i] : 0.0;
+            if (k > 0)
+              for (int m = 0; m < nd; ++m)
+                s += A[i * nd + m] * J[m * nd + j];
+            N[i * nd + j] = s;
+          }
+      J = N;
+    }
+  std::vector<float> out(nd * nd);
+  for (int i = 0; i < nd * nd; ++i)
+    out[i] = float(J[i]);
+  return out;
+}
+
 bool g_tables_ready[64] = {};
 
 int ensure_tables()
@@ -85,11 +139,15 @@ int ensure_tables()
   if (g_tables_ready[dev])
     return OC_OK;
   DegreeTables t[4];
+  Basis<float> tf[4];
   for (int p = 1; p <= 4; ++p)
-    t[p - 1] = make_tables(p);
+    {
+      t[p - 1] = make_tables(p);
+      tf[p - 1] = to_float(t[p - 1]);
+    }
   int rc;
-  if ((rc = upload_tables_d1(t)) || (rc = upload_tables_d2(t)) || (rc = upload_tables_d3(t)) ||
-      (rc = upload_tables_d4(t)))
+  if ((rc = upload_tables_d1(t, tf)) || (rc = upload_tables_d2(t, tf)) ||
+      (rc = upload_tables_d3(t, tf)) || (rc = upload_tables_d4(t, tf)))
     return rc;
   g_tables_ready[dev] = true;
   return OC_OK;
@@ -237,6 +295,7 @@ void free_all(oc_dg *op)
   f(op->d_bz);
   f(op->d_bvals);
   f(op->d_sweeps_blk);
+  f(op->d_jmat);
   f(op->d_count);
 }
 
@@ -297,10 +356,10 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       set_error("oc_dg_create: degree must be in 1..4");
       return OC_ERR_ARGUMENT;
     }
-  if (prm->precision != OC_PRECISION_FP64)
+  if (prm->precision != OC_PRECISION_FP64 && prm->precision != OC_PRECISION_MIXED)
     {
-      set_error("oc_dg_create: requested precision is not built");
-      return OC_ERR_UNSUPPORTED;
+      set_error("oc_dg_create: unknown precision");
+      return OC_ERR_ARGUMENT;
     }
   if (prm->max_iteration_height < 1)
     {
@@ -472,6 +531,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
   if ((rc = upload(&op->d_hxy, hxy)) || (rc = upload(&op->d_nbr, nbr)) ||
       (rc = upload(&op->d_fid, fid)) || (rc = upload(&op->d_finfo, finfo)) ||
       (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps_blk, (size_t)NB)) ||
+      (rc = upload(&op->d_jmat, jacobi_matrix(op->P, op->max_it))) ||
       (rc = dalloc(&op->d_count, 1)))
     {
       free_all(op);
@@ -654,6 +714,9 @@ int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
       case OC_STORE_CELL_GAUSS:
         {
           const int np = op->nq * op->nq;
+          op->zb_gauss_host.resize((size_t)N * np);
+          for (size_t i = 0; i < op->zb_gauss_host.size(); ++i)
+            op->zb_gauss_host[i] = values[i * 5];
           for (int f = 0; f < 5 && !rc; ++f)
             {
               std::vector<double> cm((size_t)N * np);
@@ -876,6 +939,74 @@ int oc_dg_project_collocation(const oc_dg *op, int32_t n_comp, const double *val
               U[i + n1 * j] = s;
             }
       }
+  return OC_OK;
+}
+
+int oc_dg_compute_errors(const oc_dg *op, const double *zeta, const double *q,
+                         const double *exact, double err[7])
+{
+  if (!op || !zeta || !q || !exact || !err)
+    return OC_ERR_ARGUMENT;
+  const int n1 = op->n1, nd = op->nd, m = op->nq, np = m * m;
+  const Rule1D nodes = gauss_lobatto(n1), rq = gauss(m);
+  std::vector<double> S(m * n1);
+  for (int a = 0; a < m; ++a)
+    for (int i = 0; i < n1; ++i)
+      S[a * n1 + i] = lagrange_value(nodes.x, i, rq.x[a]);
+  auto eval = [&](const double *U, double *out) {
+    std::vector<double> T(m * n1);
+    for (int j = 0; j < n1; ++j)
+      for (int a = 0; a < m; ++a)
+        {
+          double s = 0;
+          for (int i = 0; i < n1; ++i)
+            s += S[a * n1 + i] * U[i + n1 * j];
+          T[a + m * j] = s;
+        }
+    for (int b = 0; b < m; ++b)
+      for (int a = 0; a < m; ++a)
+        {
+          double s = 0;
+          for (int j = 0; j < n1; ++j)
+            s += S[b * n1 + j] * T[a + m * j];
+          out[a + m * b] = s;
+        }
+  };
+  std::vector<double> z(np), qx(np), qy(np);
+  double e2[3] = {0, 0, 0}, einf[3] = {0, 0, 0};
+  for (int c = 0; c < op->N; ++c)
+    {
+      const double hx = op->geom[4 * c + 2], hy = op->geom[4 * c + 3];
+      eval(zeta + (size_t)c * nd, z.data());
+      eval(q + (size_t)c * 2 * nd, qx.data());
+      eval(q + (size_t)c * 2 * nd + nd, qy.data());
+      for (int b = 0; b < m; ++b)
+        for (int a = 0; a < m; ++a)
+          {
+            const int qp = a + m * b;
+            const double JxW = rq.w[a] * rq.w[b] * hx * hy;
+            const double zb =
+              op->zb_gauss_host.empty() ? 0.0 : op->zb_gauss_host[(size_t)c * np + qp];
+            const double *ex = exact + ((size_t)c * np + qp) * 3;
+            const double hex = std::max(ex[0] + zb, 1e-12);
+            const double d0 = hex - std::max(z[qp] + zb, 1e-12);
+            const double d1 = ex[1] * hex - qx[qp];
+            const double d2 = ex[2] * hex - qy[qp];
+            e2[0] += d0 * d0 * JxW;
+            e2[1] += d1 * d1 * JxW;
+            e2[2] += d2 * d2 * JxW;
+            einf[0] = std::max(einf[0], std::fabs(d0));
+            einf[1] = std::max(einf[1], std::fabs(d1));
+            einf[2] = std::max(einf[2], std::fabs(d2));
+          }
+    }
+  err[0] = std::sqrt(e2[0]);
+  err[1] = std::sqrt(e2[1] + e2[2]);
+  err[2] = std::sqrt(e2[1]);
+  err[3] = std::sqrt(e2[2]);
+  err[4] = einf[0];
+  err[5] = einf[1];
+  err[6] = einf[2];
   return OC_OK;
 }
 

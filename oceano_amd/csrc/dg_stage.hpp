@@ -1,21 +1,51 @@
-// dg_stage_fp64.hpp -- FP64 kernels of the stage (the reference arithmetic).
+// dg_stage.hpp -- gfx950 kernels of one RK stage, FP64 or mixed precision.
 //
-//   k_height_rhs    L_zeta: local_apply_cell_height + face_height + boundary_face_height
-//                   (OceanDG.h:670-693, 937-982, 1106-1214)
-//   k_discharge     L_q: local_apply_cell_discharge + face_discharge + boundary_face_discharge
-//                   (OceanDG.h:696-743, 985-1050, 1217-1339), fused with the tensor-product
-//                   inverse mass (1533-1571) and the post-loop RK update (1878-1909)
-//   k_height_jacobi front = sum_j a_j L_zeta,j + M h(zeta^n) (829-855, 1835-1847) fused
-//                   with the capped lumped-Jacobi depth-mass solve (1374-1507)
+//   height_operator   L_zeta: local_apply_cell_height + face_height + boundary_face_height
+//                     (OceanDG.h:670-693, 937-982, 1106-1214)
+//   discharge_block   L_q: local_apply_cell_discharge + face_discharge +
+//                     boundary_face_discharge (OceanDG.h:696-743, 985-1050, 1217-1339),
+//                     fused with the tensor-product inverse mass (1533-1571) and the
+//                     post-loop RK update (1878-1909)
+//   jacobi_solve      front = sum_j a_j L_zeta,j + M h(zeta^n) (829-855, 1835-1847) and the
+//                     capped lumped-Jacobi depth-mass solve (1374-1507)
+//   k_stage           the whole stage in one launch (height and discharge work items)
+//
+// R is the arithmetic type of the cell and face integrals: double reproduces the
+// reference arithmetic; float is the mixed-precision path.  In the mixed path the state,
+// the RK combinations, the bathymetry data and the depth solve of any cell that is not
+// safely wet stay FP64; the free surface enters the FP32 integrals as a deviation from
+// the cell's first nodal value, so a flat free surface gives an exactly zero gradient and
+// an exactly zero pressure jump (the lake at rest stays exactly balanced).
+//
+// This header expects the including translation unit to define
+//   static __constant__ Basis<double> c_tab[4];  static __constant__ Basis<float> c_tabf[4];
 #pragma once
 
 #include <cfloat>
+#include <type_traits>
 
 #include "dg_kernels.hpp"
 
 namespace oceano {
 
+template <typename R>
+__device__ __forceinline__ const Basis<R> &basis(int p);
+template <>
+__device__ __forceinline__ const Basis<double> &basis<double>(int p)
+{
+  return c_tab[p - 1];
+}
+template <>
+__device__ __forceinline__ const Basis<float> &basis<float>(int p)
+{
+  return c_tabf[p - 1];
+}
 
+template <typename R>
+constexpr bool centred()
+{
+  return std::is_same<R, float>::value;
+}
 
 // exterior state on a boundary face (OceanDG.h:1130-1203 / 1246-1317); returns at_outflow
 template <typename T>
@@ -67,25 +97,29 @@ __device__ __forceinline__ bool boundary_state(const Phys<T> &M, int kind, T zm,
     }
 }
 
-// Face terms of one cell: for every face point, `inner` / `bdry` return NC values that are
-// multiplied by JxW and integrated against this cell's face basis into R[NC][nd].
-// own-cell data accessors: registers (stride 1) or the lane-interleaved LDS copy
+// own-cell data in the lane-interleaved LDS copy
+template <typename R>
 struct LdsVec
 {
-  const double *p;  // &lds[component base + lane]
-  __device__ __forceinline__ double operator[](int k) const { return p[k * CB]; }
+  const R *p;  // &lds[component base + lane]
+  __device__ __forceinline__ R operator[](int k) const { return p[k * CB]; }
 };
 
-template <int P, int NC, typename ZA, typename QA, typename InnerFn, typename BdryFn>
-__device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &T, int batch,
-                                           int lane, double hx, double hy, const ZA &Z,
-                                           const QA &QX, const QA &QY,
+// Face terms of one cell: for every face point, `inner` / `bdry` return NC values that are
+// multiplied by JxW and integrated against this cell's face basis into Racc[NC][nd].
+// Own data: Z (free-surface deviation from zc), QX, QY; neighbour traces are gathered
+// from the FP64 vectors and centred on the neighbour's first nodal value.
+template <int P, int NC, typename R, typename ZA, typename QA, typename InnerFn, typename BdryFn>
+__device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, int batch,
+                                           int lane, R hx, R hy, const ZA &Z, const QA &QX,
+                                           const QA &QY, double zc,
                                            const double *__restrict__ src_h,
                                            const double *__restrict__ src_q,
-                                           double (&R)[NC][(P + 1) * (P + 1)], InnerFn inner,
+                                           R (&Racc)[NC][(P + 1) * (P + 1)], InnerFn inner,
                                            BdryFn bdry)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P), NG = P + 2;
+  const R zcr = R(zc);
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     {
@@ -94,8 +128,8 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
       const int info = O.finfo[fo * CB + lane];
       const int kind = info & 3;
       const int nsub = (info >> 2) & 1;
-      const double flen = (f < 2) ? hy : hx;
-      double zo_l[N1], xo_l[N1], yo_l[N1];
+      const R flen = (f < 2) ? hy : hx;
+      R zo_l[N1], xo_l[N1], yo_l[N1];
 #pragma unroll
       for (int k = 0; k < N1; ++k)
         {
@@ -107,12 +141,12 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
         {
           const int bf = O.fid[(fo * 2) * CB + lane];
           const int bk = O.bkind[bf];
-          const Vec2<double> n = {f == 0 ? -1.0 : f == 1 ? 1.0 : 0.0,
-                                  f == 2 ? -1.0 : f == 3 ? 1.0 : 0.0};
+          const Vec2<R> n = {f == 0 ? R(-1) : f == 1 ? R(1) : R(0),
+                             f == 2 ? R(-1) : f == 3 ? R(1) : R(0)};
 #pragma unroll 1
           for (int a = 0; a < NQ; ++a)
             {
-              double zo = 0, xo = 0, yo = 0;
+              R zo = 0, xo = 0, yo = 0;
 #pragma unroll
               for (int k = 0; k < N1; ++k)
                 {
@@ -120,25 +154,26 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
                   xo += T.Sq[a][k] * xo_l[k];
                   yo += T.Sq[a][k] * yo_l[k];
                 }
-              const double zb = fld(O.bdryz, (size_t)bf * NQ + a);
+              zo += zcr;
+              const R zb = R(fld(O.bdryz, (size_t)bf * NQ + a));
               const double *bv = O.bvals + ((size_t)bf * NQ + a) * 3;
-              double val[NC];
-              bdry(bk, zo, Vec2<double>{xo, yo}, n, zb, bv, val);
-              const double JxW = T.wq[a] * flen;
+              R val[NC];
+              bdry(bk, zo, Vec2<R>{xo, yo}, n, zb, bv, val);
+              const R JxW = T.wq[a] * flen;
 #pragma unroll
               for (int cc = 0; cc < NC; ++cc)
                 {
-                  const double v = val[cc] * JxW;
+                  const R v = val[cc] * JxW;
 #pragma unroll
                   for (int k = 0; k < N1; ++k)
-                    R[cc][face_dof<N1>(f, k)] += T.Sq[a][k] * v;
+                    Racc[cc][face_dof<N1>(f, k)] += T.Sq[a][k] * v;
                 }
             }
         }
       else
         {
           const bool is_m = (f & 1);  // canonical orientation: minus side owns faces 1, 3
-          const Vec2<double> n = {f < 2 ? 1.0 : 0.0, f < 2 ? 0.0 : 1.0};
+          const Vec2<R> n = {f < 2 ? R(1) : R(0), f < 2 ? R(0) : R(1)};
 #pragma unroll
           for (int s = 0; s < 2; ++s)
             {
@@ -148,26 +183,28 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
               const int c2 = O.nbr[(fo * 2 + s) * CB + lane];
               const int fi = O.fid[(fo * 2 + s) * CB + lane];
               const int b2 = c2 / CB, l2 = c2 % CB;
-              double zn_l[N1], xn_l[N1], yn_l[N1];
+              const double zc2 = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
+              R zn_l[N1], xn_l[N1], yn_l[N1];
 #pragma unroll
               for (int k = 0; k < N1; ++k)
                 {
                   const int d = face_dof<N1>(f ^ 1, k);
-                  zn_l[k] = src_h[vix(b2, 1, 0, ND, d, l2)];
-                  xn_l[k] = src_q[vix(b2, 2, 0, ND, d, l2)];
-                  yn_l[k] = src_q[vix(b2, 2, 1, ND, d, l2)];
+                  zn_l[k] = R(src_h[vix(b2, 1, 0, ND, d, l2)] - zc2);
+                  xn_l[k] = R(src_q[vix(b2, 2, 0, ND, d, l2)]);
+                  yn_l[k] = R(src_q[vix(b2, 2, 1, ND, d, l2)]);
                 }
-              const double len = (kind == FK_COARSE) ? 0.5 * flen : flen;
+              const R zc2r = R(zc2);
+              const R len = (kind == FK_COARSE) ? R(0.5) * flen : flen;
 #pragma unroll 1
               for (int a = 0; a < NG; ++a)
                 {
-                  double mo[N1];
-                  double zo = 0, xo = 0, yo = 0, zn = 0, xn = 0, yn = 0;
+                  R mo[N1];
+                  R zo = 0, xo = 0, yo = 0, zn = 0, xn = 0, yn = 0;
 #pragma unroll
                   for (int k = 0; k < N1; ++k)
                     {
                       mo[k] = (kind == FK_COARSE) ? T.Sgs[s][a][k] : T.Sg[a][k];
-                      const double mn =
+                      const R mn =
                         (kind == FK_FINE) ? (nsub ? T.Sgs[1][a][k] : T.Sgs[0][a][k]) : T.Sg[a][k];
                       zo += mo[k] * zo_l[k];
                       xo += mo[k] * xo_l[k];
@@ -176,19 +213,20 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
                       xn += mn * xn_l[k];
                       yn += mn * yn_l[k];
                     }
-                  const double zbm = fld(O.facez, ((size_t)fi * NG + a) * 2 + 0);
-                  const double zbp = fld(O.facez, ((size_t)fi * NG + a) * 2 + 1);
-                  double val[NC];
-                  inner(is_m, zo, Vec2<double>{xo, yo}, zn, Vec2<double>{xn, yn}, n, zbm, zbp,
-                        val);
-                  const double JxW = T.wg[a] * len;
+                  zo += zcr;
+                  zn += zc2r;
+                  const R zbm = R(fld(O.facez, ((size_t)fi * NG + a) * 2 + 0));
+                  const R zbp = R(fld(O.facez, ((size_t)fi * NG + a) * 2 + 1));
+                  R val[NC];
+                  inner(is_m, zo, Vec2<R>{xo, yo}, zn, Vec2<R>{xn, yn}, n, zbm, zbp, val);
+                  const R JxW = T.wg[a] * len;
 #pragma unroll
                   for (int cc = 0; cc < NC; ++cc)
                     {
-                      const double v = val[cc] * JxW;
+                      const R v = val[cc] * JxW;
 #pragma unroll
                       for (int k = 0; k < N1; ++k)
-                        R[cc][face_dof<N1>(f, k)] += mo[k] * v;
+                        Racc[cc][face_dof<N1>(f, k)] += mo[k] * v;
                     }
                 }
             }
@@ -196,18 +234,18 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const DegreeTables &
     }
 }
 
-// in place: U <- M^-1 U for one component on an affine cell of area `area`
-template <int P>
-__device__ __forceinline__ void tensor_inverse(const DegreeTables &T, double *U, double inv_area)
+// in place: U <- M^-1 U for one component on an affine cell of area 1/inv_area
+template <int P, typename R>
+__device__ __forceinline__ void tensor_inverse(const Basis<R> &T, R *U, R inv_area)
 {
   constexpr int N1 = P + 1;
-  double X[N1 * N1], Y[N1 * N1];
+  R X[N1 * N1], Y[N1 * N1];
 #pragma unroll
   for (int j = 0; j < N1; ++j)
 #pragma unroll
     for (int a = 0; a < N1; ++a)
       {
-        double s = 0;
+        R s = 0;
 #pragma unroll
         for (int i = 0; i < N1; ++i)
           s += T.Ti[i][a] * U[i + N1 * j];
@@ -218,7 +256,7 @@ __device__ __forceinline__ void tensor_inverse(const DegreeTables &T, double *U,
 #pragma unroll
     for (int a = 0; a < N1; ++a)
       {
-        double s = 0;
+        R s = 0;
 #pragma unroll
         for (int j = 0; j < N1; ++j)
           s += T.Ti[j][b] * X[a + N1 * j];
@@ -229,7 +267,7 @@ __device__ __forceinline__ void tensor_inverse(const DegreeTables &T, double *U,
 #pragma unroll
     for (int i = 0; i < N1; ++i)
       {
-        double s = 0;
+        R s = 0;
 #pragma unroll
         for (int a = 0; a < N1; ++a)
           s += T.Ti[i][a] * Y[a + N1 * b];
@@ -240,7 +278,7 @@ __device__ __forceinline__ void tensor_inverse(const DegreeTables &T, double *U,
 #pragma unroll
     for (int i = 0; i < N1; ++i)
       {
-        double s = 0;
+        R s = 0;
 #pragma unroll
         for (int b = 0; b < N1; ++b)
           s += T.Ti[j][b] * X[i + N1 * b];
@@ -249,30 +287,30 @@ __device__ __forceinline__ void tensor_inverse(const DegreeTables &T, double *U,
 }
 
 // ---------------------------------------------------------------------------------------
-// L_zeta of one cell (cell term + faces) into R; Z, QX, QY: the cell's own data
+// L_zeta of one cell (cell term + faces) into Racc
 // ---------------------------------------------------------------------------------------
-template <int P, typename ZA, typename QA>
-__device__ __forceinline__ void height_operator(const OpArgs &O, const DegreeTables &T,
-                                                const Phys<double> &M, int batch, int lane,
-                                                double hx, double hy, const ZA &Z, const QA &QX,
-                                                const QA &QY, const double *__restrict__ src_h,
+template <int P, typename R, typename ZA, typename QA>
+__device__ __forceinline__ void height_operator(const OpArgs &O, const Basis<R> &T,
+                                                const Phys<R> &M, int batch, int lane, R hx,
+                                                R hy, const ZA &Z, const QA &QX, const QA &QY,
+                                                double zc, const double *__restrict__ src_h,
                                                 const double *__restrict__ src_q,
-                                                double (&R)[1][(P + 1) * (P + 1)])
+                                                R (&Racc)[1][(P + 1) * (P + 1)])
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
-  const double ihx = 1.0 / hx, ihy = 1.0 / hy, area = hx * hy;
+  const R ihx = R(1) / hx, ihy = R(1) / hy, area = hx * hy;
 #pragma unroll
   for (int k = 0; k < ND; ++k)
-    R[0][k] = 0.0;
+    Racc[0][k] = 0;
   // cell term: integral of grad(phi) . q  (mass_flux = q, ShallowWater.h:352-357)
 #pragma unroll 1
   for (int b = 0; b < NQ; ++b)
     {
-      double xs[N1], ys[N1], WS[N1], WD[N1];
+      R xs[N1], ys[N1], WS[N1], WD[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i)
         {
-          double sx = 0, sy = 0;
+          R sx = 0, sy = 0;
 #pragma unroll
           for (int j = 0; j < N1; ++j)
             {
@@ -287,15 +325,15 @@ __device__ __forceinline__ void height_operator(const OpArgs &O, const DegreeTab
 #pragma unroll
       for (int a = 0; a < NQ; ++a)
         {
-          double qx = 0, qy = 0;
+          R qx = 0, qy = 0;
 #pragma unroll
           for (int i = 0; i < N1; ++i)
             {
               qx += T.Sq[a][i] * xs[i];
               qy += T.Sq[a][i] * ys[i];
             }
-          const double JxW = T.wq[a] * T.wq[b] * area;
-          const double gx = qx * ihx * JxW, gy = qy * ihy * JxW;
+          const R JxW = T.wq[a] * T.wq[b] * area;
+          const R gx = qx * ihx * JxW, gy = qy * ihy * JxW;
 #pragma unroll
           for (int i = 0; i < N1; ++i)
             {
@@ -307,130 +345,145 @@ __device__ __forceinline__ void height_operator(const OpArgs &O, const DegreeTab
       for (int j = 0; j < N1; ++j)
 #pragma unroll
         for (int i = 0; i < N1; ++i)
-          R[0][i + N1 * j] += T.Sq[b][j] * WS[i] + T.Dq[b][j] * WD[i];
+          Racc[0][i + N1 * j] += T.Sq[b][j] * WS[i] + T.Dq[b][j] * WD[i];
     }
   phase_barrier();
-  face_terms<P, 1>(
-    O, T, batch, lane, hx, hy, Z, QX, QY, src_h, src_q, R,
-    [&](bool is_m, double zo, Vec2<double> qo, double zn, Vec2<double> qn, Vec2<double> n,
-        double zbm, double zbp, double *val) {
+  face_terms<P, 1, R>(
+    O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
+    [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
       if (is_m)
         val[0] = -M.mass_flux(zo, zn, qo, qn, n, zbm, zbp);
       else
         val[0] = M.mass_flux(zn, zo, qn, qo, n, zbm, zbp);
     },
-    [&](int bk, double zo, Vec2<double> qo, Vec2<double> n, double zb, const double *bv,
-        double *val) {
-      double zp;
-      Vec2<double> qp;
+    [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
+      R zp;
+      Vec2<R> qp;
       boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
       val[0] = -M.mass_flux(zo, zp, qo, qp, n, zb, zb);
     });
 }
 
+// own data of a cell in registers: Z as deviation from zc (zc = 0 for FP64)
+template <int P, typename R>
+__device__ __forceinline__ double load_own(const double *__restrict__ src_h,
+                                           const double *__restrict__ src_q, int batch,
+                                           int lane, R *Z, R *QX, R *QY)
+{
+  constexpr int ND = (P + 1) * (P + 1);
+  const double zc = centred<R>() ? src_h[vix(batch, 1, 0, ND, 0, lane)] : 0.0;
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+    {
+      Z[k] = R(src_h[vix(batch, 1, 0, ND, k, lane)] - zc);
+      QX[k] = R(src_q[vix(batch, 2, 0, ND, k, lane)]);
+      QY[k] = R(src_q[vix(batch, 2, 1, ND, k, lane)]);
+    }
+  return zc;
+}
+
+template <typename R>
+__device__ __forceinline__ Phys<R> physics(const OpArgs &O)
+{
+  return Phys<R>{R(O.g), R(O.cu4), R(O.nu)};
+}
+
 // L_zeta (INVERT: M^-1 L_zeta, for apply())
-template <int P, bool INVERT>
+template <int P, bool INVERT, typename R>
 __global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__restrict__ src_h,
                                                    const double *__restrict__ src_q,
                                                    double *__restrict__ dst)
 {
   constexpr int ND = (P + 1) * (P + 1);
-  const DegreeTables &T = c_tab[P - 1];
+  const Basis<R> &T = basis<R>(P);
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
   const int lane = threadIdx.x;
   if (batch * CB + lane >= O.n_cells)
     return;
-  const Phys<double> M{O.g, O.cu4, O.nu};
-  const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
-  const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
-  double Z[ND], QX[ND], QY[ND];
-#pragma unroll
-  for (int k = 0; k < ND; ++k)
-    {
-      Z[k] = src_h[vix(batch, 1, 0, ND, k, lane)];
-      QX[k] = src_q[vix(batch, 2, 0, ND, k, lane)];
-      QY[k] = src_q[vix(batch, 2, 1, ND, k, lane)];
-    }
-  double R[1][ND];
-  height_operator<P>(O, T, M, batch, lane, hx, hy, (const double *)Z, (const double *)QX,
-                     (const double *)QY, src_h, src_q, R);
+  const Phys<R> M = physics<R>(O);
+  const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
+  const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
+  R Z[ND], QX[ND], QY[ND];
+  const double zc = load_own<P, R>(src_h, src_q, batch, lane, Z, QX, QY);
+  R Racc[1][ND];
+  height_operator<P, R>(O, T, M, batch, lane, hx, hy, (const R *)Z, (const R *)QX,
+                        (const R *)QY, zc, src_h, src_q, Racc);
   if (INVERT)
-    tensor_inverse<P>(T, R[0], 1.0 / (hx * hy));
+    tensor_inverse<P, R>(T, Racc[0], R(1) / (hx * hy));
 #pragma unroll
   for (int k = 0; k < ND; ++k)
-    dst[vix(batch, 1, 0, ND, k, lane)] = R[0][k];
+    dst[vix(batch, 1, 0, ND, k, lane)] = double(Racc[0][k]);
 }
 
 // ---------------------------------------------------------------------------------------
-// L_q of one cell; MODE 0: stage (M^-1, RK update), 1: apply (M^-1 -> dst), 2: rhs
-// The cell's own zeta, q (and velocity) are staged in LDS, lane-interleaved.
+// L_q of one batch; MODE 0: stage (M^-1, RK update), 1: apply (M^-1 -> dst), 2: rhs
+// The batch's own zeta, q (and velocity) are staged in LDS, lane-interleaved.
 // ---------------------------------------------------------------------------------------
 template <int P, bool VISC>
-constexpr int discharge_lds_doubles()
+constexpr int discharge_lds_words()
 {
   return (VISC ? 5 : 3) * (P + 1) * (P + 1) * CB;
 }
 
-template <int P, bool VISC, int MODE>
+template <int P, bool VISC, int MODE, typename R>
 __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs &S, int batch,
-                                                double *lds)
+                                                R *lds)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
-  const DegreeTables &T = c_tab[P - 1];
+  const Basis<R> &T = basis<R>(P);
   const int lane = threadIdx.x;
   const double *__restrict__ src_h = S.src_h;
   const double *__restrict__ src_q = S.src_q;
-  // stage the batch's own data: contiguous, coalesced
-  {
-    const double *gh = src_h + (size_t)batch * ND * CB;
-    const double *gq = src_q + (size_t)batch * 2 * ND * CB;
-#pragma unroll
-    for (int k = 0; k < ND; ++k)
-      {
-        lds[k * CB + lane] = gh[k * CB + lane];
-        lds[(ND + k) * CB + lane] = gq[k * CB + lane];
-        lds[(2 * ND + k) * CB + lane] = gq[(ND + k) * CB + lane];
-        if (VISC)
-          {
-            const double *gv = S.src_v + (size_t)batch * 2 * ND * CB;
-            lds[(3 * ND + k) * CB + lane] = gv[k * CB + lane];
-            lds[(4 * ND + k) * CB + lane] = gv[(ND + k) * CB + lane];
-          }
-      }
-  }
-  // each lane reads only its own column of the staged data: no barrier needed beyond the
-  // wave's own ordering of its LDS writes and reads
-  __builtin_amdgcn_wave_barrier();
-  if (batch * CB + lane >= O.n_cells)
-    return;
-  const LdsVec Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
-  const LdsVec UX{lds + 3 * ND * CB + lane}, UY{lds + 4 * ND * CB + lane};
-  const Phys<double> M{O.g, O.cu4, O.nu};
-  const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
-  const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
-  const double ihx = 1.0 / hx, ihy = 1.0 / hy, area = hx * hy;
-  double R[2][ND];
+  const double *gh = src_h + (size_t)batch * ND * CB;
+  const double *gq = src_q + (size_t)batch * 2 * ND * CB;
+  const double zc = centred<R>() ? gh[lane] : 0.0;
+  // stage the batch's own data: contiguous, coalesced; each lane later reads only its own
+  // column, so no barrier is needed
 #pragma unroll
   for (int k = 0; k < ND; ++k)
     {
-      R[0][k] = 0.0;
-      R[1][k] = 0.0;
+      lds[k * CB + lane] = R(gh[k * CB + lane] - zc);
+      lds[(ND + k) * CB + lane] = R(gq[k * CB + lane]);
+      lds[(2 * ND + k) * CB + lane] = R(gq[(ND + k) * CB + lane]);
+      if (VISC)
+        {
+          const double *gv = S.src_v + (size_t)batch * 2 * ND * CB;
+          lds[(3 * ND + k) * CB + lane] = R(gv[k * CB + lane]);
+          lds[(4 * ND + k) * CB + lane] = R(gv[(ND + k) * CB + lane]);
+        }
+    }
+  if (batch * CB + lane >= O.n_cells)
+    return;
+  const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
+  const LdsVec<R> UX{lds + 3 * ND * CB + lane}, UY{lds + 4 * ND * CB + lane};
+  const Phys<R> M = physics<R>(O);
+  const R g = R(O.g), nu = R(O.nu);
+  const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
+  const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
+  const R ihx = R(1) / hx, ihy = R(1) / hy, area = hx * hy;
+  const R zcr = R(zc);
+  R Racc[2][ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+    {
+      Racc[0][k] = 0;
+      Racc[1][k] = 0;
     }
   const size_t c0base = (size_t)batch * NQ * NQ * CB + lane;
 #pragma unroll 1
   for (int b = 0; b < NQ; ++b)
     {
-      double zs[N1], zd[N1], xs[N1], ys[N1];
-      double uxs[VISC ? N1 : 1], uxd[VISC ? N1 : 1], uys[VISC ? N1 : 1], uyd[VISC ? N1 : 1];
-      double WSX[N1], WDX[N1], WSY[N1], WDY[N1];
+      R zs[N1], zd[N1], xs[N1], ys[N1];
+      R uxs[VISC ? N1 : 1], uxd[VISC ? N1 : 1], uys[VISC ? N1 : 1], uyd[VISC ? N1 : 1];
+      R WSX[N1], WDX[N1], WSY[N1], WDY[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i)
         {
-          double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+          R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
           for (int j = 0; j < N1; ++j)
             {
-              const double zv = Z[i + N1 * j];
+              const R zv = Z[i + N1 * j];
               s0 += T.Sq[b][j] * zv;
               s1 += T.Dq[b][j] * zv;
               s2 += T.Sq[b][j] * QX[i + N1 * j];
@@ -442,7 +495,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
           ys[i] = s3;
           if (VISC)
             {
-              double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+              R t0 = 0, t1 = 0, t2 = 0, t3 = 0;
 #pragma unroll
               for (int j = 0; j < N1; ++j)
                 {
@@ -461,7 +514,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
 #pragma unroll
       for (int a = 0; a < NQ; ++a)
         {
-          double z = 0, zx = 0, zy = 0, qx = 0, qy = 0;
+          R z = 0, zx = 0, zy = 0, qx = 0, qy = 0;
 #pragma unroll
           for (int i = 0; i < N1; ++i)
             {
@@ -471,18 +524,19 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
               qx += T.Sq[a][i] * xs[i];
               qy += T.Sq[a][i] * ys[i];
             }
+          z += zcr;
           zx *= ihx;
           zy *= ihy;
           const int q = a + NQ * b;
           const size_t pi = c0base + (size_t)q * CB;
-          const double zb = fld(O.cell0[0], pi);
-          const double h = M.depth(z, zb);
-          const double s = M.inv_depth(h);
-          const double vx = qx * s, vy = qy * s;
-          double F00 = qx * vx, F01 = qx * vy, F10 = qy * vx, F11 = qy * vy;
+          const R zb = R(fld(O.cell0[0], pi));
+          const R h = M.depth(z, zb);
+          const R s = M.inv_depth(h);
+          const R vx = qx * s, vy = qy * s;
+          R F00 = qx * vx, F01 = qx * vy, F10 = qy * vx, F11 = qy * vy;
           if (VISC)
             {
-              double G00 = 0, G01 = 0, G10 = 0, G11 = 0;
+              R G00 = 0, G01 = 0, G10 = 0, G11 = 0;
 #pragma unroll
               for (int i = 0; i < N1; ++i)
                 {
@@ -495,37 +549,37 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
               G01 = h * (G01 * ihy);
               G10 = h * (G10 * ihx);
               G11 = h * (G11 * ihy);
-              const double div = G00 + G11;
-              F00 = F00 - O.nu * (G00 + G00) + O.nu * div;
-              F01 = F01 - O.nu * (G01 + G10);
-              F10 = F10 - O.nu * (G10 + G01);
-              F11 = F11 - O.nu * (G11 + G11) + O.nu * div;
+              const R div = G00 + G11;
+              F00 = F00 - nu * (G00 + G00) + nu * div;
+              F01 = F01 - nu * (G01 + G10);
+              F10 = F10 - nu * (G10 + G01);
+              F11 = F11 - nu * (G11 + G11) + nu * div;
             }
-          const double mgh = -O.g * h;
-          double Sx = mgh * zx, Sy = mgh * zy;
+          const R mgh = -g * h;
+          R Sx = mgh * zx, Sy = mgh * zy;
           if (O.has_friction)
             {
-              const double n_m = fld(O.cell0[1], pi);
-              const double inv_d = 1. / (exp(log(h) * 0.33333333333));
-              const double hcf = O.g * n_m * n_m * inv_d * sqrt(vx * vx + vy * vy);
+              const R n_m = R(fld(O.cell0[1], pi));
+              const R inv_d = R(1) / (exp(log(h) * R(0.33333333333)));
+              const R hcf = g * n_m * n_m * inv_d * sqrt(vx * vx + vy * vy);
               Sx -= hcf * vx;
               Sy -= hcf * vy;
             }
           if (O.has_wind)
             {
-              Sx += fld(O.cell0[2], pi);
-              Sy += fld(O.cell0[3], pi);
+              Sx += R(fld(O.cell0[2], pi));
+              Sy += R(fld(O.cell0[3], pi));
             }
           if (O.has_coriolis)
             {
-              const double fc = fld(O.cell0[4], pi);
+              const R fc = R(fld(O.cell0[4], pi));
               Sx += fc * qy;
               Sy += -fc * qx;
             }
-          const double JxW = T.wq[a] * T.wq[b] * area;
-          const double g0x = F00 * ihx * JxW, g0y = F01 * ihy * JxW;
-          const double g1x = F10 * ihx * JxW, g1y = F11 * ihy * JxW;
-          const double v0 = Sx * JxW, v1 = Sy * JxW;
+          const R JxW = T.wq[a] * T.wq[b] * area;
+          const R g0x = F00 * ihx * JxW, g0y = F01 * ihy * JxW;
+          const R g1x = F10 * ihx * JxW, g1y = F11 * ihy * JxW;
+          const R v0 = Sx * JxW, v1 = Sy * JxW;
 #pragma unroll
           for (int i = 0; i < N1; ++i)
             {
@@ -540,41 +594,39 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
 #pragma unroll
         for (int i = 0; i < N1; ++i)
           {
-            R[0][i + N1 * j] += T.Sq[b][j] * WSX[i] + T.Dq[b][j] * WDX[i];
-            R[1][i + N1 * j] += T.Sq[b][j] * WSY[i] + T.Dq[b][j] * WDY[i];
+            Racc[0][i + N1 * j] += T.Sq[b][j] * WSX[i] + T.Dq[b][j] * WDX[i];
+            Racc[1][i + N1 * j] += T.Sq[b][j] * WSY[i] + T.Dq[b][j] * WDY[i];
           }
     }
   phase_barrier();
-  face_terms<P, 2>(
-    O, T, batch, lane, hx, hy, Z, QX, QY, src_h, src_q, R,
-    [&](bool is_m, double zo, Vec2<double> qo, double zn, Vec2<double> qn, Vec2<double> n,
-        double zbm, double zbp, double *val) {
+  face_terms<P, 2, R>(
+    O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
+    [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
       if (is_m)
         {
-          const Vec2<double> F = M.adv_flux(zo, zn, qo, qn, n, zbm, zbp);
-          const Vec2<double> Pm = M.pres_flux(zo, zn, n, zbm, zbp);
+          const Vec2<R> F = M.adv_flux(zo, zn, qo, qn, n, zbm, zbp);
+          const Vec2<R> Pm = M.pres_flux(zo, zn, n, zbm, zbp);
           val[0] = -F.x - Pm.x;
           val[1] = -F.y - Pm.y;
         }
       else
         {
-          const Vec2<double> F = M.adv_flux(zn, zo, qn, qo, n, zbm, zbp);
-          const Vec2<double> Pp = M.pres_flux(zo, zn, n, zbp, zbm);
+          const Vec2<R> F = M.adv_flux(zn, zo, qn, qo, n, zbm, zbp);
+          const Vec2<R> Pp = M.pres_flux(zo, zn, n, zbp, zbm);
           val[0] = F.x + Pp.x;
           val[1] = F.y + Pp.y;
         }
     },
-    [&](int bk, double zo, Vec2<double> qo, Vec2<double> n, double zb, const double *bv,
-        double *val) {
-      double zp;
-      Vec2<double> qp;
+    [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
+      R zp;
+      Vec2<R> qp;
       const bool at_outflow = boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
-      Vec2<double> F = M.adv_flux(zo, zp, qo, qp, n, zb, zb);
-      const Vec2<double> Pr = M.pres_flux(zo, zp, n, zb, zb);
+      Vec2<R> F = M.adv_flux(zo, zp, qo, qp, n, zb, zb);
+      const Vec2<R> Pr = M.pres_flux(zo, zp, n, zb, zb);
       F.x += Pr.x;
       F.y += Pr.y;
-      if (at_outflow && (qo.x * n.x + qo.y * n.y) < -1e-12)
-        F = {0.0, 0.0};
+      if (at_outflow && (qo.x * n.x + qo.y * n.y) < R(-1e-12))
+        F = {R(0), R(0)};
       val[0] = -F.x;
       val[1] = -F.y;
     });
@@ -584,7 +636,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int k = 0; k < ND; ++k)
-          S.dst[vix(batch, 2, c, ND, k, lane)] = R[c][k];
+          S.dst[vix(batch, 2, c, ND, k, lane)] = double(Racc[c][k]);
       return;
     }
   if (MODE == 0 && S.scratch)
@@ -593,44 +645,45 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int k = 0; k < ND; ++k)
-          S.scratch[vix(batch, 2, c, ND, k, lane)] = R[c][k];
+          S.scratch[vix(batch, 2, c, ND, k, lane)] = double(Racc[c][k]);
     }
   phase_barrier();
-  const double inv_area = 1.0 / area;
-  tensor_inverse<P>(T, R[0], inv_area);
-  tensor_inverse<P>(T, R[1], inv_area);
+  const R inv_area = R(1) / area;
+  tensor_inverse<P, R>(T, Racc[0], inv_area);
+  tensor_inverse<P, R>(T, Racc[1], inv_area);
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
     for (int k = 0; k < ND; ++k)
-      S.dst[vix(batch, 2, c, ND, k, lane)] = R[c][k];
+      S.dst[vix(batch, 2, c, ND, k, lane)] = double(Racc[c][k]);
   if (MODE == 0)
     {
-      // post-op of OceanDG.h:1878-1909: dst = w + sum_j a_j k_{j+1}, k_{stage+1} = R
+      // post-op of OceanDG.h:1878-1909: dst = w + sum_j a_j k_{j+1}, k_{stage+1} = Racc
 #pragma unroll
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int k = 0; k < ND; ++k)
           {
             const size_t idx = vix(batch, 2, c, ND, k, lane);
-            double acc = S.wsol[idx] + S.factor[0] * (S.stage == 0 ? R[c][k] : S.kprev[0][idx]);
+            const double kk = double(Racc[c][k]);
+            double acc = S.wsol[idx] + S.factor[0] * (S.stage == 0 ? kk : S.kprev[0][idx]);
             for (int j = 1; j <= S.stage; ++j)
-              acc += S.factor[j] * (j == S.stage ? R[c][k] : S.kprev[j][idx]);
+              acc += S.factor[j] * (j == S.stage ? kk : S.kprev[j][idx]);
             S.rk_dst[idx] = acc;
           }
     }
 }
 
-template <int P, bool VISC, int MODE>
+template <int P, bool VISC, int MODE, typename R>
 __global__ __launch_bounds__(CB) void k_discharge(OpArgs O, StageArgs S)
 {
-  __shared__ double lds[discharge_lds_doubles<P, VISC>()];
-  discharge_block<P, VISC, MODE>(O, S, logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle),
-                                 lds);
+  __shared__ R lds[discharge_lds_words<P, VISC>()];
+  discharge_block<P, VISC, MODE, R>(
+    O, S, logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle), lds);
 }
 
 // ---------------------------------------------------------------------------------------
-// front + capped lumped-Jacobi depth-mass solve
+// front + capped lumped-Jacobi depth-mass solve (FP64, the reference arithmetic)
 // ---------------------------------------------------------------------------------------
 // R += sgn * integral(phi * depth(Z)) at GL(ng)^2; optionally counts dry points
 template <int P, bool COUNT>
@@ -790,6 +843,85 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
   return k;
 }
 
+// Mixed precision depth solve.  When the cell is wet at every GL point with a margin the
+// depth floor cannot bind and the capped Jacobi iteration is linear in the increment:
+// delta^K = sum_{k<K} (I - D^-1 M)^k D^-1 r with r = sum_j a_j L_zeta,j, a fixed matrix
+// (on the unit cell) times r / area.  Cells near the floor take the FP64 sweep.
+// Returns the sweep count reported for the cell (max_it on the closed form).
+template <int P>
+__device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTables &T,
+                                                 const Phys<double> &M, int batch, int lane,
+                                                 double area, const double *r, double *Z)
+{
+  constexpr int N1 = P + 1, ND = N1 * N1, NG = P + 2;
+  const Basis<float> &Tf = basis<float>(P);
+  // minimum depth of zeta^n over the GL points (FP32 on the deviation from node 0)
+  const double zc = Z[0];
+  float dev[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    dev[i] = float(Z[i] - zc);
+  float hmin = FLT_MAX;
+  const size_t zbase = (size_t)batch * NG * NG * CB + lane;
+#pragma unroll 1
+  for (int b = 0; b < NG; ++b)
+    {
+      float zs[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i)
+        {
+          float s = 0;
+#pragma unroll
+          for (int j = 0; j < N1; ++j)
+            s += Tf.Sg[b][j] * dev[i + N1 * j];
+          zs[i] = s;
+        }
+#pragma unroll
+      for (int a = 0; a < NG; ++a)
+        {
+          float z = 0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i)
+            z += Tf.Sg[a][i] * zs[i];
+          const double zb = fld(O.cell1, zbase + (size_t)(a + NG * b) * CB);
+          const float h = float(zc + zb) + z;
+          hmin = h < hmin ? h : hmin;
+        }
+    }
+  float rf[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    rf[i] = float(r[i]);
+  const float ia = float(1.0 / area);
+  float delta[ND];
+  float dmax = 0.f;
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    {
+      float s = 0.f;
+#pragma unroll
+      for (int j = 0; j < ND; ++j)
+        s += O.jmat[i * ND + j] * rf[j];
+      delta[i] = s * ia;
+      dmax = fmaxf(dmax, fabsf(delta[i]));
+    }
+  if (hmin > float(O.wet_margin) * dmax && hmin > 1e-6f)
+    {
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        Z[i] += double(delta[i]);
+      return O.max_it;
+    }
+  // FP64 reference path: front = r + M h(zeta^n), then the capped sweeps
+  double F[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    F[i] = r[i];
+  unsigned long long bits = 0;
+  depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
+  return jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
+}
+
 __device__ __forceinline__ void store_sweeps(unsigned int *blk, int batch, int k)
 {
   if (!blk)
@@ -804,15 +936,15 @@ __device__ __forceinline__ void store_sweeps(unsigned int *blk, int batch, int k
 
 // front from ki_h[1..stage+1] (all in memory) and the Jacobi solve: the path used when
 // the stage's outputs alias its inputs
-template <int P>
+template <int P, typename R>
 __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
 {
-  constexpr int N1 = P + 1, ND = N1 * N1;
-  const DegreeTables &T = c_tab[P - 1];
+  constexpr int ND = (P + 1) * (P + 1);
+  const DegreeTables &T = basis<double>(P);
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
   const int lane = threadIdx.x;
   const bool active = batch * CB + lane < O.n_cells;
-  const Phys<double> M{O.g, O.cu4, O.nu};
+  const Phys<double> M = physics<double>(O);
   int k = 0;
   if (active)
     {
@@ -830,32 +962,36 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
           F[i] = f;
           Z[i] = S.wsol[idx];
         }
-      unsigned long long bits = 0;
-      depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
-      if (S.scratch)
+      if (centred<R>())
+        k = mixed_depth_solve<P>(O, T, M, batch, lane, area, F, Z);
+      else
+        {
+          unsigned long long bits = 0;
+          depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
+          if (S.scratch)
 #pragma unroll
-        for (int i = 0; i < ND; ++i)
-          S.scratch[vix(batch, 1, 0, ND, i, lane)] = F[i];
-      k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
+            for (int i = 0; i < ND; ++i)
+              S.scratch[vix(batch, 1, 0, ND, i, lane)] = F[i];
+          k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
+        }
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         S.rk_dst[vix(batch, 1, 0, ND, i, lane)] = Z[i];
     }
-  (void)N1;
   store_sweeps(S.sweeps_blk, batch, k);
 }
 
-// L_zeta, front and Jacobi solve of one batch of cells in one pass: L_zeta stays in
+// L_zeta, front and depth solve of one batch of cells in one pass: L_zeta stays in
 // registers for the front (it is also written out as ki_h[stage+1] for later stages)
-template <int P>
+template <int P, typename R>
 __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageArgs &S,
                                                    int batch)
 {
   constexpr int ND = (P + 1) * (P + 1);
-  const DegreeTables &T = c_tab[P - 1];
+  const DegreeTables &T = basis<double>(P);
   const int lane = threadIdx.x;
   const bool active = batch * CB + lane < O.n_cells;
-  const Phys<double> M{O.g, O.cu4, O.nu};
+  const Phys<double> M = physics<double>(O);
   int k = 0;
   if (active)
     {
@@ -864,23 +1000,18 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
       const double area = hx * hy;
       double F[ND];
       {
-        double Z[ND], QX[ND], QY[ND];
-#pragma unroll
-        for (int i = 0; i < ND; ++i)
-          {
-            Z[i] = S.src_h[vix(batch, 1, 0, ND, i, lane)];
-            QX[i] = S.src_q[vix(batch, 2, 0, ND, i, lane)];
-            QY[i] = S.src_q[vix(batch, 2, 1, ND, i, lane)];
-          }
-        double R[1][ND];
-        height_operator<P>(O, T, M, batch, lane, hx, hy, (const double *)Z,
-                           (const double *)QX, (const double *)QY, S.src_h, S.src_q, R);
+        R Zo[ND], QX[ND], QY[ND];
+        const double zc = load_own<P, R>(S.src_h, S.src_q, batch, lane, Zo, QX, QY);
+        R Racc[1][ND];
+        height_operator<P, R>(O, basis<R>(P), physics<R>(O), batch, lane, R(hx), R(hy),
+                              (const R *)Zo, (const R *)QX, (const R *)QY, zc, S.src_h,
+                              S.src_q, Racc);
 #pragma unroll
         for (int i = 0; i < ND; ++i)
           {
             const size_t idx = vix(batch, 1, 0, ND, i, lane);
-            S.dst[idx] = R[0][i];  // ki_h[stage+1]
-            const double kk = R[0][i];
+            const double kk = double(Racc[0][i]);
+            S.dst[idx] = kk;  // ki_h[stage+1]
             double f = S.factor[0] * (S.stage == 0 ? kk : S.kprev[0][idx]);
             for (int j = 1; j <= S.stage; ++j)
               f += S.factor[j] * (j == S.stage ? kk : S.kprev[j][idx]);
@@ -892,13 +1023,18 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         Z[i] = S.wsol[vix(batch, 1, 0, ND, i, lane)];
-      unsigned long long bits = 0;
-      depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
-      if (S.scratch)
+      if (centred<R>())
+        k = mixed_depth_solve<P>(O, T, M, batch, lane, area, F, Z);
+      else
+        {
+          unsigned long long bits = 0;
+          depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
+          if (S.scratch)
 #pragma unroll
-        for (int i = 0; i < ND; ++i)
-          S.scratch[vix(batch, 1, 0, ND, i, lane)] = F[i];
-      k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
+            for (int i = 0; i < ND; ++i)
+              S.scratch[vix(batch, 1, 0, ND, i, lane)] = F[i];
+          k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
+        }
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         S.rk_dst[vix(batch, 1, 0, ND, i, lane)] = Z[i];
@@ -908,15 +1044,15 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
 
 // the whole stage in one launch: work items w = (batch, kind) are dealt so that each XCD
 // holds a contiguous range of batches; kind 0 = height stage, kind 1 = discharge stage
-template <int P, bool VISC>
+template <int P, bool VISC, typename R>
 __global__ __launch_bounds__(CB) void k_stage(OpArgs O, StageArgs H, StageArgs Q)
 {
-  __shared__ double lds[discharge_lds_doubles<P, VISC>()];
+  __shared__ R lds[discharge_lds_words<P, VISC>()];
   const int w = logical_batch(blockIdx.x, 2 * O.n_batches, O.xcd_swizzle);
   if (w & 1)
-    discharge_block<P, VISC, 0>(O, Q, w >> 1, lds);
+    discharge_block<P, VISC, 0, R>(O, Q, w >> 1, lds);
   else
-    height_stage_block<P>(O, H, w >> 1);
+    height_stage_block<P, R>(O, H, w >> 1);
 }
 
 }  // namespace oceano

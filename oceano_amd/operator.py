@@ -183,6 +183,18 @@ class OceanoOperator:
         check(lib().oc_dg_rhs(self._h, _vp(src[0]), _vp(src[1]), _vp(vel), _vp(dst[0]),
                               _vp(dst[1]), _stream()), "oc_dg_rhs")
 
+    def compute_errors(self, zeta, q, exact, time):
+        """OceanDG.h:2382-2459 per field: {depth, discharge, hu, hv} L2 and Linf errors of
+        operator-order vectors against exact(xy, t) -> [n,3] (free surface, u, v)."""
+        zh = np.ascontiguousarray(self.unpack(zeta, 1).cpu().numpy())
+        qh = np.ascontiguousarray(self.unpack(q, 2).cpu().numpy())
+        ex = np.ascontiguousarray(exact(self.quadrature_points("cell_0"), time))
+        err = np.zeros(7)
+        check(lib().oc_dg_compute_errors(self._h, dptr(zh), dptr(qh), dptr(ex), dptr(err)),
+              "oc_dg_compute_errors")
+        return {"l2_depth": err[0], "l2_discharge": err[1], "l2_hu": err[2], "l2_hv": err[3],
+                "linf_depth": err[4], "linf_hu": err[5], "linf_hv": err[6]}
+
     def count_nonfinite(self, vec):
         n = np.zeros(1, np.int64)
         check(lib().oc_dg_count_nonfinite(self._h, _vp(vec), vec.numel(),

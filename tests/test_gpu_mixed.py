@@ -1,0 +1,122 @@
+"""Mixed-precision path (FP32 cell/face integrals, FP64 state / RK / depth floor) against
+the FP64 CPU oracle.
+
+Bars: the lake at rest stays exactly balanced; one-stage operators agree with FP64 to
+FP32 accuracy; over a run the solution agrees with the FP64 solution to 1e-6 relative L2
+per field (the matched-error target) and its errors against the exact vortex match the
+FP64 errors.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import make_pair, rel_l2, to_dev, to_host
+from oceano_amd import problems
+from oceano_amd.integrator import butcher_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mixed_lake_at_rest_is_exact(cuda):
+    """Same bar as the FP64 path: the projected flat surface carries 1e-16 round-off, and
+    the FP32 integrals on cell-centred deviations keep L at that level (an uncentred FP32
+    evaluation leaves ~1e-7)."""
+    for degree in (1, 2, 3, 4):
+        P = problems.lake_at_rest(degree=degree)
+        op, o, h, q = make_pair(P, precision="mixed")
+        Lh, Lq = op.initialize_vector(0), op.initialize_vector(1)
+        op.rhs([to_dev(op, h, 1), to_dev(op, q, 2)], [Lh, Lq])
+        torch.cuda.synchronize()
+        print("  p=%d max|L_zeta| %.3e max|L_q| %.3e" % (degree, float(Lh.abs().max()),
+                                                       float(Lq.abs().max())))
+        assert float(Lh.abs().max()) < 1e-13 and float(Lq.abs().max()) < 1e-13
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+def test_mixed_rhs_fp32_accuracy(cuda, degree):
+    P = problems.vortex(degree=degree, refine=1, refine_fraction=0.3, seed=11)
+    op, o, h, q = make_pair(P, precision="mixed")
+    rng = np.random.default_rng(degree)
+    h = h + 1e-3 * rng.standard_normal(h.shape)
+    q = q + 1e-2 * rng.standard_normal(q.shape)
+    Lh_ref, Lq_ref = o.rhs(h, q)
+    Lh, Lq = op.initialize_vector(0), op.initialize_vector(1)
+    op.rhs([to_dev(op, h, 1), to_dev(op, q, 2)], [Lh, Lq])
+    torch.cuda.synchronize()
+    eh, eq = rel_l2(to_host(op, Lh, 1), Lh_ref), rel_l2(to_host(op, Lq, 2), Lq_ref)
+    print("  mixed rhs rel L2: L_zeta %.3e L_q %.3e" % (eh, eq))
+    assert eh < 2e-5 and eq < 2e-5
+
+
+def _run(op, h, q, rows, n_steps):
+    S = len(rows)
+    w_h, w_q = to_dev(op, h, 1), to_dev(op, q, 2)
+    ki_h = [op.initialize_vector(0) for _ in range(S + 1)]
+    ki_q = [op.initialize_vector(1) for _ in range(S + 1)]
+    r_h = [op.initialize_vector(0) for _ in range(2)]
+    r_q = [op.initialize_vector(1) for _ in range(2)]
+    for _ in range(n_steps):
+        for s in range(S):
+            ri = [w_h, w_q] if s == 0 else [r_h[s % 2], r_q[s % 2]]
+            op.perform_stage_hydro(s, 0.0, rows[s][:s + 1], ri + [None, None], ki_h, ki_q,
+                                   w_h, w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
+    torch.cuda.synchronize()
+    return w_h, w_q
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_mixed_vortex_run_matches_fp64(cuda, degree):
+    """50 RK4 steps of the vortex on a hanging-node mesh: the mixed solution differs from
+    the FP64 oracle's by < 1e-6 relative L2 per field, and both have the same error
+    against the exact travelling vortex."""
+    P = problems.vortex(degree=degree, refine=2, refine_fraction=0.3, seed=20261202)
+    op, o, h, q = make_pair(P, precision="mixed")
+    dt = 2.8e-4 / degree ** 1.5
+    n = 50
+    rows, _ = butcher_rows("stage_4_order_4", dt)
+    mh, mq = _run(op, h, q, rows, n)
+    S = len(rows)
+    w_h, w_q = h.copy(), q.copy()
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(S + 1)]
+    r_h, r_q = o.zeros(1), o.zeros(2)
+    for _ in range(n):
+        o.time_step(rows, w_h, w_q, ki_h, ki_q, r_h, r_q)
+    gh, gq = to_host(op, mh, 1), to_host(op, mq, 2)
+    nd = o.nd
+    qx = lambda v: v.reshape(-1, 2, nd)[:, 0].ravel()
+    qy = lambda v: v.reshape(-1, 2, nd)[:, 1].ravel()
+    d = {"zeta": rel_l2(gh, w_h), "hu": rel_l2(qx(gq), qx(w_q)), "hv": rel_l2(qy(gq), qy(w_q)),
+         "d_zeta": rel_l2(gh - h, w_h - h)}
+    e_mixed = op.compute_errors(mh, mq, P.exact, n * dt)
+    e_fp64 = o.errors(w_h, w_q, P.exact, n * dt)
+    print("  mixed vs fp64:", " ".join("%s=%.2e" % kv for kv in d.items()))
+    print("  L2 error vs exact: mixed depth %.4e hu %.4e hv %.4e | fp64 depth %.4e hu %.4e "
+          "hv %.4e" % (e_mixed["l2_depth"], e_mixed["l2_hu"], e_mixed["l2_hv"], e_fp64[0],
+                       e_fp64[2], e_fp64[3]))
+    assert d["zeta"] < 1e-6 and d["hu"] < 1e-6 and d["hv"] < 1e-6
+    # matched error: equal to 0.1%, above an FP32 floor of 1e-8 on O(1) fields
+    for k, i in (("l2_depth", 0), ("l2_hu", 2), ("l2_hv", 3)):
+        assert abs(e_mixed[k] - e_fp64[i]) <= 1e-3 * e_fp64[i] + 1e-8
+
+
+def test_mixed_wet_dry_takes_fp64_depth_solve(cuda):
+    """Dam break onto a dry bed: cells near the floor take the FP64 sweep; the run stays
+    finite and close to the FP64 oracle."""
+    import test_gpu_parity as T
+    P = T._dambreak_problem(2)
+    op, o, h, q = make_pair(P, precision="mixed")
+    rows, _ = butcher_rows("stage_4_order_4", 2e-3)
+    mh, mq = _run(op, h, q, rows, 3)
+    S = len(rows)
+    w_h, w_q = h.copy(), q.copy()
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(S + 1)]
+    r_h, r_q = o.zeros(1), o.zeros(2)
+    for _ in range(3):
+        o.time_step(rows, w_h, w_q, ki_h, ki_q, r_h, r_q)
+    gh, gq = to_host(op, mh, 1), to_host(op, mq, 2)
+    assert np.isfinite(gh).all() and np.isfinite(gq).all()
+    eh, eq = rel_l2(gh, w_h), rel_l2(gq, w_q)
+    print("  dambreak mixed vs fp64: zeta %.2e q %.2e" % (eh, eq))
+    assert eh < 1e-5 and eq < 1e-3
