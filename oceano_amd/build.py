@@ -62,7 +62,21 @@ def build(verbose=False, extra=(), incremental=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_cpp_test(verbose)
     return LIB
+
+
+CPP_TEST = os.path.join(ROOT, "tests", "cpp", "test_adapter")
+
+
+def build_cpp_test(verbose=False):
+    """The C++ drop-in test links the library like a reference-side caller would."""
+    src = CPP_TEST + ".cpp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950", "-I" + os.path.join(ROOT, "include"),
+           src, "-L" + HERE, "-loceano_dg", "-Wl,-rpath,$ORIGIN/../../oceano_amd", "-o", CPP_TEST]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

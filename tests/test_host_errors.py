@@ -1,0 +1,60 @@
+"""Error behaviour of the C ABI that is decided on the host before any GPU work: status
+codes instead of the reference's exceptions (OceanDG.h:1199-1203, IcbcBase.h:146-160)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oceano_amd import _capi
+from oceano_amd._capi import lib
+from oceano_amd.mesh import Mesh
+
+
+def _create(mesh, ids=(), kinds=(), degree=2, precision=0, max_it=15):
+    ids = np.array(ids, np.int32)
+    kinds = np.array(kinds, np.int32)
+    desc = _capi.MeshDesc(mesh.n_cells, _capi.dptr(mesh.geom), mesh.inner.shape[0],
+                          _capi.iptr(mesh.inner), mesh.boundary.shape[0],
+                          _capi.iptr(mesh.boundary))
+    prm = _capi.Params(degree, 9.81, 0.1, 0.0, max_it, precision, len(ids), _capi.iptr(ids),
+                       _capi.iptr(kinds))
+    h = C.c_void_p()
+    rc = lib().oc_dg_create(C.byref(desc), C.byref(prm), C.byref(h))
+    return rc, lib().oc_last_error().decode()
+
+
+def test_unknown_boundary_id():
+    m = Mesh.rectangle(0, 1, 0, 1, 3, 2, boundary_ids=(1, 2, 0, 0))
+    rc, msg = _create(m, ids=[0, 1], kinds=[_capi.OC_BC_WALL, _capi.OC_BC_WALL])
+    assert rc == _capi.OC_ERR_BOUNDARY and "Unknown boundary id 2" in msg
+
+
+def test_duplicate_boundary_assignment():
+    m = Mesh.rectangle(0, 1, 0, 1, 3, 2, boundary_ids=(0, 0, 0, 0))
+    rc, msg = _create(m, ids=[0, 0], kinds=[_capi.OC_BC_WALL, _capi.OC_BC_HEIGHT_INFLOW])
+    assert rc == _capi.OC_ERR_BOUNDARY and "already set" in msg
+
+
+@pytest.mark.parametrize("degree,max_it,precision", [(0, 15, 0), (5, 15, 0), (2, 0, 0),
+                                                     (2, 15, 7)])
+def test_bad_parameters(degree, max_it, precision):
+    m = Mesh.rectangle(0, 1, 0, 1, 2, 2, periodic=(True, True))
+    rc, _ = _create(m, degree=degree, max_it=max_it, precision=precision)
+    assert rc == _capi.OC_ERR_ARGUMENT
+
+
+def test_inconsistent_topology():
+    m = Mesh.rectangle(0, 1, 0, 1, 3, 3, periodic=(True, True))
+    m.inner = np.ascontiguousarray(m.inner[:-1])  # one face missing
+    rc, msg = _create(m)
+    assert rc == _capi.OC_ERR_MESH and "not covered" in msg
+    m = Mesh.rectangle(0, 1, 0, 1, 3, 3, periodic=(True, True))
+    m.inner = np.ascontiguousarray(m.inner.copy())
+    m.inner[0, 4] = 3  # joins non-opposite faces
+    rc, msg = _create(m)
+    assert rc == _capi.OC_ERR_MESH
+
+
+def test_null_arguments():
+    assert lib().oc_dg_create(None, None, None) == _capi.OC_ERR_ARGUMENT
+    assert lib().oc_dg_destroy(None) == _capi.OC_ERR_ARGUMENT
