@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this size 3 times (a kernel of "
+                         "known HBM traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -155,6 +158,16 @@ def main():
                 "nonfinite": op.count_nonfinite(w_h) + op.count_nonfinite(w_q),
                 "sweeps_last_stage": op.last_sweeps(), "t_end": n_steps * args.dt}
 
+    if args.copy_mib:
+        n = args.copy_mib * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {p: run(p) for p in precisions}
     head = runs["mixed"] if "mixed" in runs else runs["fp64"]

@@ -121,7 +121,8 @@ int oc_dg_destroy(oc_dg *op);
  *   OC_STORE_BOUNDARY     [n_bdry][nq]       points; 1 value: z_b
  * with nq = floor(1.5p)+1, ng = p+2, n1 = p+1 and points lexicographic (x fastest).
  * oc_dg_quadrature_points writes 2 doubles (x, y) per point to host memory; the caller
- * evaluates its functions there and hands the values to oc_dg_set_static_data (host). */
+ * evaluates its functions there and hands the values to oc_dg_set_static_data (host).
+ * A store with no points (no boundary faces, no inner faces) accepts null pointers. */
 #define OC_STORE_CELL_GAUSS 0
 #define OC_STORE_CELL_LOBATTO 1
 #define OC_STORE_CELL_COLLOC 2

@@ -23,6 +23,12 @@ namespace oceano {
 thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+static int bad_argument(const char *fn)
+{
+  set_error(std::string(fn) + ": null pointer or invalid size");
+  return OC_ERR_ARGUMENT;
+}
+
 namespace {
 
 DegreeTables make_tables(int p)
@@ -555,7 +561,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
 int oc_dg_destroy(oc_dg *op)
 {
   if (!op)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   free_all(op);
   delete op;
   return OC_OK;
@@ -564,7 +570,7 @@ int oc_dg_destroy(oc_dg *op)
 int oc_dg_sizes(const oc_dg *op, int64_t sizes[7])
 {
   if (!op || !sizes)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   sizes[0] = op->N;
   sizes[1] = (int64_t)op->NB * CB;
   sizes[2] = op->nd;
@@ -578,7 +584,7 @@ int oc_dg_sizes(const oc_dg *op, int64_t sizes[7])
 int oc_dg_vector_length(const oc_dg *op, int32_t n_components, int64_t *length)
 {
   if (!op || !length || n_components < 1)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   *length = (int64_t)op->NB * CB * n_components * op->nd;
   return OC_OK;
 }
@@ -586,7 +592,7 @@ int oc_dg_vector_length(const oc_dg *op, int32_t n_components, int64_t *length)
 int oc_dg_n_points(const oc_dg *op, int32_t store, int64_t *n)
 {
   if (!op || !n)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   const int64_t ni = (int64_t)op->inner.size() / 6, nb = (int64_t)op->bdry.size() / 3;
   switch (store)
     {
@@ -613,8 +619,13 @@ int oc_dg_n_points(const oc_dg *op, int32_t store, int64_t *n)
 
 int oc_dg_quadrature_points(const oc_dg *op, int32_t store, double *xy)
 {
-  if (!op || !xy)
-    return OC_ERR_ARGUMENT;
+  int64_t npts = 0;
+  if (const int rc = oc_dg_n_points(op, store, &npts))
+    return rc;
+  if (npts == 0)
+    return OC_OK;  // empty store (no boundary / inner faces): xy may be null
+  if (!xy)
+    return bad_argument(__func__);
   const int N = op->N;
   auto cell_rule = [&](const Rule1D &r) {
     const int m = (int)r.x.size();
@@ -688,8 +699,13 @@ int oc_dg_quadrature_points(const oc_dg *op, int32_t store, double *xy)
 
 int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
 {
-  if (!op || !values)
-    return OC_ERR_ARGUMENT;
+  int64_t npts = 0;
+  if (const int rc = oc_dg_n_points(op, store, &npts))
+    return rc;
+  if (npts == 0)
+    return OC_OK;  // empty store: nothing is read from it
+  if (!values)
+    return bad_argument(__func__);
   const int N = op->N, NB = op->NB;
   int rc = OC_OK;
   auto interleave = [&](const std::vector<double> &cm, int npts, double **dptr, double &cval) {
@@ -778,11 +794,13 @@ int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
 
 int oc_dg_set_boundary_values(oc_dg *op, const double *values)
 {
-  if (!op || !values)
-    return OC_ERR_ARGUMENT;
+  if (!op)
+    return bad_argument(__func__);
   const size_t n = (op->bdry.size() / 3) * op->nq * 3;
   if (n == 0)
     return OC_OK;
+  if (!values)
+    return bad_argument(__func__);
   HIP_TRY(hipMemcpy(op->d_bvals, values, n * sizeof(double), hipMemcpyHostToDevice));
   return OC_OK;
 }
@@ -791,7 +809,7 @@ int oc_dg_pack(oc_dg *op, int32_t n_components, const double *cell_major, double
                void *stream)
 {
   if (!op || n_components < 1 || !cell_major || !op_order)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   const size_t total = (size_t)op->NB * CB * n_components * op->nd;
   hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, op->N,
                      n_components, op->nd, cell_major, op_order, 0);
@@ -803,7 +821,7 @@ int oc_dg_unpack(oc_dg *op, int32_t n_components, const double *op_order, double
                  void *stream)
 {
   if (!op || n_components < 1 || !cell_major || !op_order)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   const size_t total = (size_t)op->NB * CB * n_components * op->nd;
   hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, op->N,
                      n_components, op->nd, op_order, cell_major, 1);
@@ -870,7 +888,7 @@ static int rhs_or_apply(oc_dg *op, bool invert, const double *src_h, const doubl
                         const double *src_vel, double *dst_h, double *dst_q, void *stream)
 {
   if (!op || !src_h || !src_q || !dst_h || !dst_q)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   if (op->nu != 0.0 && !src_vel)
     {
       set_error("velocity vector required when viscosity != 0");
@@ -908,7 +926,7 @@ int oc_dg_project_collocation(const oc_dg *op, int32_t n_comp, const double *val
                               double *coeffs)
 {
   if (!op || n_comp < 1 || !values || !coeffs)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   const int n1 = op->n1, nd = op->nd;
   const Rule1D nodes = gauss_lobatto(n1), rn = gauss(n1);
   std::vector<double> S(nd);
@@ -946,7 +964,7 @@ int oc_dg_compute_errors(const oc_dg *op, const double *zeta, const double *q,
                          const double *exact, double err[7])
 {
   if (!op || !zeta || !q || !exact || !err)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   const int n1 = op->n1, nd = op->nd, m = op->nq, np = m * m;
   const Rule1D nodes = gauss_lobatto(n1), rq = gauss(m);
   std::vector<double> S(m * n1);
@@ -1013,7 +1031,7 @@ int oc_dg_compute_errors(const oc_dg *op, const double *zeta, const double *q,
 int oc_dg_count_nonfinite(oc_dg *op, const double *x, int64_t n, int64_t *count, void *stream)
 {
   if (!op || !x || n < 0 || !count)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, op->d_count);
   const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
@@ -1030,7 +1048,7 @@ int oc_dg_count_nonfinite(oc_dg *op, const double *x, int64_t n, int64_t *count,
 int oc_dg_set_timing(oc_dg *op, int32_t enable)
 {
   if (!op)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   op->timing = enable != 0;
   return OC_OK;
 }
@@ -1038,7 +1056,7 @@ int oc_dg_set_timing(oc_dg *op, int32_t enable)
 int oc_dg_timing_report(oc_dg *op, double *ms, int64_t *count)
 {
   if (!op || !ms || !count)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   for (int k = 0; k < TK_COUNT; ++k)
     {
       ms[k] = 0.0;
@@ -1061,7 +1079,7 @@ int oc_dg_timing_report(oc_dg *op, double *ms, int64_t *count)
 int oc_dg_last_sweeps(oc_dg *op, int64_t *sweeps, void *stream)
 {
   if (!op || !sweeps)
-    return OC_ERR_ARGUMENT;
+    return bad_argument(__func__);
   hipStream_t s = (hipStream_t)stream;
   std::vector<unsigned int> h(op->NB);
   HIP_TRY(hipMemcpyAsync(h.data(), op->d_sweeps_blk, h.size() * sizeof(unsigned int),

@@ -96,8 +96,9 @@ int main()
       op.initialize_vector(k2h[j], 0);
       op.initialize_vector(k2q[j], 1);
     }
-  const double a[3][3] = {{0.5 * dt}, {0.0, 0.5 * dt}, {0.0, 0.0, dt}};
-  const double b[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  // the integrator scales its tableau by the step (ExplicitRungeKuttaIntegrator.h:202-205)
+  const double a[3][3] = {{0.5 * dt}, {0.0 * dt, 0.5 * dt}, {0.0 * dt, 0.0 * dt, 1.0 * dt}};
+  const double b[4] = {(1.0 / 6.0) * dt, (1.0 / 3.0) * dt, (1.0 / 3.0) * dt, (1.0 / 6.0) * dt};
   for (int n = 0; n < 10; ++n)
     for (unsigned s = 0; s < 4; ++s)
       {
@@ -126,13 +127,13 @@ int main()
 
   // errors become exceptions carrying the status code
   int code = 0;
+  oc_mesh *m2 = nullptr;
   try
     {
       const int32_t bid[1] = {7};
       const int32_t kind[1] = {OC_BC_WALL};
-      oc_mesh *m2 = nullptr;
       const int32_t ids2[4] = {1, 2, 0, 0};
-      oc_mesh_create_rectangle(0, 1, 0, 1, 2, 2, 0, 0, ids2, &m2);
+      oceano::check(oc_mesh_create_rectangle(0, 1, 0, 1, 2, 2, 0, 0, ids2, &m2), "mesh");
       int64_t i2[4];
       oc_mesh_info(m2, i2);
       std::vector<double> g2(4 * i2[0]);
@@ -145,13 +146,13 @@ int main()
       p2.boundary_ids = bid;
       p2.boundary_kinds = kind;
       oceano::OceanoOperator<DeviceVector> bad(d2, p2);
-      oc_mesh_destroy(m2);
     }
   catch (const oceano::Error &e)
     {
       code = e.code;
       std::printf("expected failure: %s\n", e.what());
     }
+  oc_mesh_destroy(m2);
   oc_mesh_destroy(mesh);
   const bool ok = same && code == OC_ERR_BOUNDARY;
   std::printf("%s\n", ok ? "OK" : "FAILED");

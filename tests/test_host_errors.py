@@ -58,3 +58,24 @@ def test_inconsistent_topology():
 def test_null_arguments():
     assert lib().oc_dg_create(None, None, None) == _capi.OC_ERR_ARGUMENT
     assert lib().oc_dg_destroy(None) == _capi.OC_ERR_ARGUMENT
+
+
+@pytest.mark.gpu
+def test_empty_store_accepts_null_pointers(cuda):
+    """A fully periodic mesh has no boundary faces: the boundary store has zero points and
+    its calls succeed with null pointers (the C++ adapter passes an empty vector's data(),
+    which is null); a non-empty store still rejects a null pointer, with a message."""
+    from oceano_amd import problems
+    from oceano_amd.operator import OceanoOperator
+    P = problems.vortex(degree=2, refine=0)
+    print("[host-errors] periodic vortex p=2, %d cells" % P.mesh.n_cells, flush=True)
+    op = OceanoOperator(P.mesh, 2)
+    n = C.c_int64(-1)
+    assert lib().oc_dg_n_points(op._h, _capi.OC_STORE_BOUNDARY, C.byref(n)) == 0
+    assert n.value == 0
+    assert lib().oc_dg_quadrature_points(op._h, _capi.OC_STORE_BOUNDARY, None) == 0
+    assert lib().oc_dg_set_static_data(op._h, _capi.OC_STORE_BOUNDARY, None) == 0
+    assert lib().oc_dg_set_boundary_values(op._h, None) == 0
+    rc = lib().oc_dg_quadrature_points(op._h, _capi.OC_STORE_CELL_GAUSS, None)
+    assert rc == _capi.OC_ERR_ARGUMENT
+    assert "null pointer" in lib().oc_last_error().decode()
