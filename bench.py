@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--slot-chunk", type=int, default=None,
+                    help="library option slot_chunk (cell grouping by face signature)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this size 3 times (a kernel of "
                          "known HBM traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -105,6 +107,9 @@ def main():
     from oceano_amd import problems
     from oceano_amd.integrator import butcher_rows
 
+    if args.slot_chunk is not None:
+        from oceano_amd import _capi
+        _capi.check(_capi.lib().oc_set_option(b"slot_chunk", args.slot_chunk), "oc_set_option")
     P = build_problem()
     rows, _ = butcher_rows("stage_4_order_4", args.dt)
     S = len(rows)

@@ -38,6 +38,14 @@ extern "C" {
 
 const char *oc_last_error(void);
 
+/* Library tuning options read by later oc_dg_create calls (no reference counterpart: the
+ * reference leaves cell batching to deal.II's MatrixFree).
+ *   "slot_chunk"  cells are grouped by face signature (which faces are the coarse side of
+ *                 two subfaces or on the boundary) within chunks of this many 64-cell
+ *                 batches of the caller's order; 0 (default) = one XCD's share of the
+ *                 batches, < 0 = keep the caller's order.  Results do not depend on it. */
+int oc_set_option(const char *name, int64_t value);
+
 /* ------------------------------------------------------------------------------------
  * Boundary-condition kinds.  Replaces ICBC::BcBase's six boundary maps
  * (IcbcBase.h:65-77): each boundary id carries exactly one kind (IcbcBase.h:146-260).

@@ -62,6 +62,7 @@ class Params(C.Structure):
 # exported symbol -> (restype, argtypes)
 SIGNATURES = {
     "oc_last_error": (C.c_char_p, []),
+    "oc_set_option": (C.c_int, [C.c_char_p, C.c_int64]),
     "oc_mesh_create_rectangle": (C.c_int, [D, D, D, D, I32, I32, I32, I32, PI32, C.POINTER(P)]),
     "oc_mesh_refine_global": (C.c_int, [P, I32]),
     "oc_mesh_refine_fraction": (C.c_int, [P, D, U64]),

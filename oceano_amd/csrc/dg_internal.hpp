@@ -29,8 +29,12 @@ struct oc_dg
   std::vector<int32_t> inner_in;  // as given
   std::vector<int32_t> bdry;
   std::map<int, int> bc_of_id;
+  // cell slots: the kernels see cell order[i] in slot i (batch i / 64, lane i % 64);
+  // pos is the inverse.  Every host-facing array stays in the caller's cell order.
+  std::vector<int32_t> order, pos;
   // device
   double *d_hxy = nullptr;
+  int *d_order = nullptr;
   int *d_nbr = nullptr, *d_fid = nullptr, *d_finfo = nullptr, *d_bkind = nullptr;
   double *d_cell0[5] = {};
   double c_cell0[5] = {};

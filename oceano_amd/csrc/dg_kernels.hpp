@@ -19,6 +19,7 @@
 namespace oceano {
 
 constexpr int CB = OC_CELL_BATCH;
+constexpr int NXCD = 8;  // MI355X: accelerator complex dies, each with its own L2
 constexpr int MAXN1 = 5, MAXNQ = 7, MAXNG = 6, MAXKI = 9;
 
 template <typename R>
@@ -33,6 +34,9 @@ struct Basis
   R Ti[MAXN1][MAXN1];      // inverse of [l_i(Gauss(n1)_a)]_{a,i}, stored [i][a]
   R wn[MAXN1];             // Gauss(n1) weights
   R lumped[MAXN1];         // sum_a Sg[a][i] wg[a]
+  R ilumped2[MAXN1][MAXN1];  // 1 / (lumped[i] lumped[j]): inverse lumped mass / area
+  R M1[MAXN1][MAXN1];        // sum_a Sg[a][i] Sg[a][j] wg[a]: 1-D GL(ng) mass (exact)
+  R lam;                     // max_{a,i} Sg[a][i]^2: bound of |phi_ij| at the GL(ng)^2 points
 };
 using DegreeTables = Basis<double>;
 
@@ -101,7 +105,7 @@ __device__ __forceinline__ int logical_batch(int b, int nb, int swz)
 {
   if (!swz)
     return b;
-  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  const int q = nb / NXCD, r = nb % NXCD, x = b % NXCD, i = b / NXCD;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
@@ -117,6 +121,11 @@ __device__ __forceinline__ void phase_barrier()
   __builtin_amdgcn_sched_barrier(0);
   asm volatile("" ::: "memory");
 }
+
+// Placed after a group of independent loads, before their first use: the machine scheduler
+// otherwise pairs each load with its use (in FP32 code with its conversion, to save
+// registers), so the wave waits out one memory latency per load instead of one per group.
+__device__ __forceinline__ void issue_barrier() { __builtin_amdgcn_sched_barrier(0); }
 
 __device__ __forceinline__ double fld(const Field &f, size_t i)
 {

@@ -74,6 +74,20 @@ DegreeTables make_tables(int p)
         s += t.Sg[a][i] * t.wg[a];
       t.lumped[i] = s;
     }
+  double smax = 0.0;
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j)
+      {
+        t.ilumped2[i][j] = 1.0 / (t.lumped[i] * t.lumped[j]);
+        double m = 0.0;
+        for (int a = 0; a < ng; ++a)
+          m += t.Sg[a][i] * t.Sg[a][j] * t.wg[a];
+        t.M1[i][j] = m;
+      }
+  for (int a = 0; a < ng; ++a)
+    for (int i = 0; i < n1; ++i)
+      smax = std::max(smax, std::fabs(t.Sg[a][i]));
+  t.lam = smax * smax;
   return t;
 }
 
@@ -186,8 +200,8 @@ int upload(T **p, const std::vector<T> &h)
   return OC_OK;
 }
 
-__global__ void k_pack(int n_cells, int ncomp, int nd, const double *__restrict__ in,
-                       double *__restrict__ out, int unpack)
+__global__ void k_pack(int n_cells, int ncomp, int nd, const int *__restrict__ order,
+                       const double *__restrict__ in, double *__restrict__ out, int unpack)
 {
   const size_t n_batches = ((size_t)n_cells + CB - 1) / CB;
   const size_t total = n_batches * CB * ncomp * nd;
@@ -201,13 +215,14 @@ __global__ void k_pack(int n_cells, int ncomp, int nd, const double *__restrict_
       const size_t r2 = r / nd;
       const int comp = r2 % ncomp;
       const size_t batch = r2 / ncomp;
-      const size_t cell = batch * CB + lane;
-      if (cell >= (size_t)n_cells)
+      const size_t slot = batch * CB + lane;
+      if (slot >= (size_t)n_cells)
         {
           if (!unpack)
             out[t] = 0.0;  // padding lanes
           continue;
         }
+      const size_t cell = (size_t)order[slot];
       const size_t cm = (cell * ncomp + comp) * nd + k;
       if (unpack)
         out[cm] = in[t];
@@ -290,6 +305,7 @@ void free_all(oc_dg *op)
       (void)hipFree(p);
   };
   f(op->d_hxy);
+  f(op->d_order);
   f(op->d_nbr);
   f(op->d_fid);
   f(op->d_finfo);
@@ -343,11 +359,77 @@ bool uniform(const std::vector<double> &v, double &val)
   return true;
 }
 
+// Cell slots.  A wave evaluates the second subface of face f whenever one of its 64 cells
+// is the coarse side there, and the boundary branch whenever one of them has a boundary
+// face f; cells are therefore grouped by their face signature (per face: conforming or
+// fine side / coarse side / boundary), keeping the caller's order inside each group, so
+// that almost every batch of 64 has one signature and runs one face path.
+// Sorting is done within chunks of consecutive cells of the caller's order: a chunk of one
+// XCD's share of the batches keeps each XCD's neighbour traces in its own L2.
+int g_slot_chunk = 0;  // batches per chunk; 0: one XCD's share; < 0: the caller's order
+
+int cell_slots(const oc_mesh_desc *mesh, std::vector<int32_t> &order, std::vector<int32_t> &pos)
+{
+  const int N = mesh->n_cells;
+  std::vector<int> code((size_t)N * 4, 0);  // 0 conforming / fine side, 1 coarse, 2 boundary
+  auto mark = [&](int c, int f, int k) {
+    if (c < 0 || c >= N || f < 0 || f > 3)
+      return false;
+    code[4 * (size_t)c + f] = std::max(code[4 * (size_t)c + f], k);
+    return true;
+  };
+  for (int i = 0; i < mesh->n_inner; ++i)
+    {
+      const int *F = &mesh->inner[6 * i];
+      // a subface index marks that side as the coarse one (its face is split in two)
+      if (!mark(F[0], F[1], F[2] >= 0 ? 1 : 0) || !mark(F[3], F[4], F[5] >= 0 ? 1 : 0))
+        {
+          set_error("inner face " + std::to_string(i) + " is malformed");
+          return OC_ERR_MESH;
+        }
+    }
+  for (int b = 0; b < mesh->n_boundary; ++b)
+    if (!mark(mesh->boundary[3 * b], mesh->boundary[3 * b + 1], 2))
+      {
+        set_error("boundary face " + std::to_string(b) + " is malformed");
+        return OC_ERR_MESH;
+      }
+  std::vector<int> sig(N);
+  for (int c = 0; c < N; ++c)
+    sig[c] = code[4 * c] + 3 * (code[4 * c + 1] + 3 * (code[4 * c + 2] + 3 * code[4 * c + 3]));
+  order.resize(N);
+  for (int c = 0; c < N; ++c)
+    order[c] = c;
+  const int nb = (N + CB - 1) / CB;
+  const long chunk = g_slot_chunk < 0 ? 0
+                     : (long)CB * (g_slot_chunk > 0 ? g_slot_chunk : (nb + NXCD - 1) / NXCD);
+  for (long c0 = 0; chunk > 0 && c0 < N; c0 += chunk)
+    std::stable_sort(order.begin() + c0, order.begin() + std::min<long>(N, c0 + chunk),
+                     [&](int a, int b) { return sig[a] < sig[b]; });
+  pos.resize(N);
+  for (int i = 0; i < N; ++i)
+    pos[order[i]] = i;
+  return OC_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char *oc_last_error(void) { return oceano::g_last_error.c_str(); }
+
+int oc_set_option(const char *name, int64_t value)
+{
+  if (!name)
+    return bad_argument(__func__);
+  if (std::strcmp(name, "slot_chunk") == 0)
+    {
+      g_slot_chunk = (int)value;
+      return OC_OK;
+    }
+  set_error(std::string("oc_set_option: unknown option ") + name);
+  return OC_ERR_ARGUMENT;
+}
 
 int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
 {
@@ -407,13 +489,22 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       op->bc_of_id[prm->boundary_ids[i]] = kind;
     }
   const int N = op->N, NB = op->NB;
-  // topology -> per-cell face tables
+  if (const int rc2 = cell_slots(mesh, op->order, op->pos))
+    {
+      delete op;
+      return rc2;
+    }
+  // topology -> per-cell face tables (cell c sits in slot pos[c])
   std::vector<int> finfo((size_t)NB * 4 * CB, -1), nbr((size_t)NB * 4 * 2 * CB, 0),
     fid((size_t)NB * 4 * 2 * CB, 0);
   std::vector<int> slots((size_t)N * 4, 0);
-  auto at4 = [&](int c, int f) { return ((size_t)(c / CB) * 4 + f) * CB + c % CB; };
+  auto at4 = [&](int c, int f) {
+    const int i = op->pos[c];
+    return ((size_t)(i / CB) * 4 + f) * CB + i % CB;
+  };
   auto at8 = [&](int c, int f, int s) {
-    return (((size_t)(c / CB) * 4 + f) * 2 + s) * CB + c % CB;
+    const int i = op->pos[c];
+    return (((size_t)(i / CB) * 4 + f) * 2 + s) * CB + i % CB;
   };
   auto fail = [&](const std::string &m, int code) {
     delete op;
@@ -458,9 +549,9 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       if (sm < 0 && sp < 0)
         {
           ok = set_kind(cm, fm, FK_CONF) && set_kind(cp, fp, FK_CONF);
-          nbr[at8(cm, fm, 0)] = cp;
+          nbr[at8(cm, fm, 0)] = op->pos[cp];
           fid[at8(cm, fm, 0)] = i;
-          nbr[at8(cp, fp, 0)] = cm;
+          nbr[at8(cp, fp, 0)] = op->pos[cm];
           fid[at8(cp, fp, 0)] = i;
           slots[4 * cm + fm] += 1;
           slots[4 * cp + fp] += 1;
@@ -468,9 +559,9 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       else if (sm >= 0)
         {
           ok = set_kind(cm, fm, FK_COARSE) && set_kind(cp, fp, FK_FINE | (sm << 2));
-          nbr[at8(cm, fm, sm)] = cp;
+          nbr[at8(cm, fm, sm)] = op->pos[cp];
           fid[at8(cm, fm, sm)] = i;
-          nbr[at8(cp, fp, 0)] = cm;
+          nbr[at8(cp, fp, 0)] = op->pos[cm];
           fid[at8(cp, fp, 0)] = i;
           slots[4 * cm + fm] += 1 << (4 + sm);
           slots[4 * cp + fp] += 1;
@@ -478,9 +569,9 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       else
         {
           ok = set_kind(cm, fm, FK_FINE | (sp << 2)) && set_kind(cp, fp, FK_COARSE);
-          nbr[at8(cm, fm, 0)] = cp;
+          nbr[at8(cm, fm, 0)] = op->pos[cp];
           fid[at8(cm, fm, 0)] = i;
-          nbr[at8(cp, fp, sp)] = cm;
+          nbr[at8(cp, fp, sp)] = op->pos[cm];
           fid[at8(cp, fp, sp)] = i;
           slots[4 * cm + fm] += 1;
           slots[4 * cp + fp] += 1 << (4 + sp);
@@ -526,15 +617,17 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
   std::vector<double> hxy((size_t)NB * 2 * CB, 1.0);
   for (int c = 0; c < N; ++c)
     {
-      hxy[((size_t)(c / CB) * 2 + 0) * CB + c % CB] = op->geom[4 * c + 2];
-      hxy[((size_t)(c / CB) * 2 + 1) * CB + c % CB] = op->geom[4 * c + 3];
+      const int i = op->pos[c];
+      hxy[((size_t)(i / CB) * 2 + 0) * CB + i % CB] = op->geom[4 * c + 2];
+      hxy[((size_t)(i / CB) * 2 + 1) * CB + i % CB] = op->geom[4 * c + 3];
     }
   if ((rc = ensure_tables()))
     {
       delete op;
       return rc;
     }
-  if ((rc = upload(&op->d_hxy, hxy)) || (rc = upload(&op->d_nbr, nbr)) ||
+  if ((rc = upload(&op->d_hxy, hxy)) || (rc = upload(&op->d_order, op->order)) ||
+      (rc = upload(&op->d_nbr, nbr)) ||
       (rc = upload(&op->d_fid, fid)) || (rc = upload(&op->d_finfo, finfo)) ||
       (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps_blk, (size_t)NB)) ||
       (rc = upload(&op->d_jmat, jacobi_matrix(op->P, op->max_it))) ||
@@ -721,7 +814,10 @@ int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
     std::vector<double> il((size_t)NB * npts * CB, 0.0);
     for (int c = 0; c < N; ++c)
       for (int q = 0; q < npts; ++q)
-        il[((size_t)(c / CB) * npts + q) * CB + c % CB] = cm[(size_t)c * npts + q];
+        {
+          const int i = op->pos[c];
+          il[((size_t)(i / CB) * npts + q) * CB + i % CB] = cm[(size_t)c * npts + q];
+        }
     cval = 0.0;
     return upload(dptr, il);
   };
@@ -812,7 +908,7 @@ int oc_dg_pack(oc_dg *op, int32_t n_components, const double *cell_major, double
     return bad_argument(__func__);
   const size_t total = (size_t)op->NB * CB * n_components * op->nd;
   hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, op->N,
-                     n_components, op->nd, cell_major, op_order, 0);
+                     n_components, op->nd, op->d_order, cell_major, op_order, 0);
   HIP_TRY(hipGetLastError());
   return OC_OK;
 }
@@ -824,7 +920,7 @@ int oc_dg_unpack(oc_dg *op, int32_t n_components, const double *op_order, double
     return bad_argument(__func__);
   const size_t total = (size_t)op->NB * CB * n_components * op->nd;
   hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, op->N,
-                     n_components, op->nd, op_order, cell_major, 1);
+                     n_components, op->nd, op->d_order, op_order, cell_major, 1);
   HIP_TRY(hipGetLastError());
   return OC_OK;
 }

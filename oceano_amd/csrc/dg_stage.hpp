@@ -53,7 +53,7 @@ __device__ __forceinline__ bool boundary_state(const Phys<T> &M, int kind, T zm,
                                                Vec2<T> n, T zb, const double *bv, T &zp,
                                                Vec2<T> &qp)
 {
-  const T un = qm.x * n.x + qm.y * n.y;
+  const T un = dot_n(qm, n);
   switch (kind)
     {
       case OC_BC_WALL:
@@ -183,15 +183,24 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
               const int c2 = O.nbr[(fo * 2 + s) * CB + lane];
               const int fi = O.fid[(fo * 2 + s) * CB + lane];
               const int b2 = c2 / CB, l2 = c2 % CB;
+              double zg[N1], xg[N1], yg[N1];
               const double zc2 = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
-              R zn_l[N1], xn_l[N1], yn_l[N1];
 #pragma unroll
               for (int k = 0; k < N1; ++k)
                 {
                   const int d = face_dof<N1>(f ^ 1, k);
-                  zn_l[k] = R(src_h[vix(b2, 1, 0, ND, d, l2)] - zc2);
-                  xn_l[k] = R(src_q[vix(b2, 2, 0, ND, d, l2)]);
-                  yn_l[k] = R(src_q[vix(b2, 2, 1, ND, d, l2)]);
+                  zg[k] = src_h[vix(b2, 1, 0, ND, d, l2)];
+                  xg[k] = src_q[vix(b2, 2, 0, ND, d, l2)];
+                  yg[k] = src_q[vix(b2, 2, 1, ND, d, l2)];
+                }
+              issue_barrier();
+              R zn_l[N1], xn_l[N1], yn_l[N1];
+#pragma unroll
+              for (int k = 0; k < N1; ++k)
+                {
+                  zn_l[k] = R(zg[k] - zc2);
+                  xn_l[k] = R(xg[k]);
+                  yn_l[k] = R(yg[k]);
                 }
               const R zc2r = R(zc2);
               const R len = (kind == FK_COARSE) ? R(0.5) * flen : flen;
@@ -371,13 +380,22 @@ __device__ __forceinline__ double load_own(const double *__restrict__ src_h,
                                            int lane, R *Z, R *QX, R *QY)
 {
   constexpr int ND = (P + 1) * (P + 1);
-  const double zc = centred<R>() ? src_h[vix(batch, 1, 0, ND, 0, lane)] : 0.0;
+  double zg[ND], xg[ND], yg[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k)
     {
-      Z[k] = R(src_h[vix(batch, 1, 0, ND, k, lane)] - zc);
-      QX[k] = R(src_q[vix(batch, 2, 0, ND, k, lane)]);
-      QY[k] = R(src_q[vix(batch, 2, 1, ND, k, lane)]);
+      zg[k] = src_h[vix(batch, 1, 0, ND, k, lane)];
+      xg[k] = src_q[vix(batch, 2, 0, ND, k, lane)];
+      yg[k] = src_q[vix(batch, 2, 1, ND, k, lane)];
+    }
+  issue_barrier();
+  const double zc = centred<R>() ? zg[0] : 0.0;
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+    {
+      Z[k] = R(zg[k] - zc);
+      QX[k] = R(xg[k]);
+      QY[k] = R(yg[k]);
     }
   return zc;
 }
@@ -436,20 +454,44 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   const double *__restrict__ src_q = S.src_q;
   const double *gh = src_h + (size_t)batch * ND * CB;
   const double *gq = src_q + (size_t)batch * 2 * ND * CB;
-  const double zc = centred<R>() ? gh[lane] : 0.0;
   // stage the batch's own data: contiguous, coalesced; each lane later reads only its own
   // column, so no barrier is needed
+  double zc;
+  {
+    double zg[ND], xg[ND], yg[ND];
 #pragma unroll
-  for (int k = 0; k < ND; ++k)
+    for (int k = 0; k < ND; ++k)
+      {
+        zg[k] = gh[k * CB + lane];
+        xg[k] = gq[k * CB + lane];
+        yg[k] = gq[(ND + k) * CB + lane];
+      }
+    issue_barrier();
+    zc = centred<R>() ? zg[0] : 0.0;
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+      {
+        lds[k * CB + lane] = R(zg[k] - zc);
+        lds[(ND + k) * CB + lane] = R(xg[k]);
+        lds[(2 * ND + k) * CB + lane] = R(yg[k]);
+      }
+  }
+  if (VISC)
     {
-      lds[k * CB + lane] = R(gh[k * CB + lane] - zc);
-      lds[(ND + k) * CB + lane] = R(gq[k * CB + lane]);
-      lds[(2 * ND + k) * CB + lane] = R(gq[(ND + k) * CB + lane]);
-      if (VISC)
+      const double *gv = S.src_v + (size_t)batch * 2 * ND * CB;
+      double ug[ND], vg[ND];
+#pragma unroll
+      for (int k = 0; k < ND; ++k)
         {
-          const double *gv = S.src_v + (size_t)batch * 2 * ND * CB;
-          lds[(3 * ND + k) * CB + lane] = R(gv[k * CB + lane]);
-          lds[(4 * ND + k) * CB + lane] = R(gv[(ND + k) * CB + lane]);
+          ug[k] = gv[k * CB + lane];
+          vg[k] = gv[(ND + k) * CB + lane];
+        }
+      issue_barrier();
+#pragma unroll
+      for (int k = 0; k < ND; ++k)
+        {
+          lds[(3 * ND + k) * CB + lane] = R(ug[k]);
+          lds[(4 * ND + k) * CB + lane] = R(vg[k]);
         }
     }
   if (batch * CB + lane >= O.n_cells)
@@ -625,7 +667,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       const Vec2<R> Pr = M.pres_flux(zo, zp, n, zb, zb);
       F.x += Pr.x;
       F.y += Pr.y;
-      if (at_outflow && (qo.x * n.x + qo.y * n.y) < R(-1e-12))
+      if (at_outflow && dot_n(qo, n) < R(-1e-12))
         F = {R(0), R(0)};
       val[0] = -F.x;
       val[1] = -F.y;
@@ -658,19 +700,34 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       S.dst[vix(batch, 2, c, ND, k, lane)] = double(Racc[c][k]);
   if (MODE == 0)
     {
-      // post-op of OceanDG.h:1878-1909: dst = w + sum_j a_j k_{j+1}, k_{stage+1} = Racc
+      // post-op of OceanDG.h:1878-1909: dst = w + sum_j a_j k_{j+1}, k_{stage+1} = Racc,
+      // summed in the reference's order; the loop over the earlier k vectors is outside
+      // the loop over the DoFs so that the 2 nd loads of one vector are in flight together
+      const size_t base = (size_t)batch * 2 * ND * CB + lane;
+      double acc[2 * ND];
+#pragma unroll
+      for (int i = 0; i < 2 * ND; ++i)
+        acc[i] = S.wsol[base + (size_t)i * CB];
+      issue_barrier();
+      for (int j = 0; j < S.stage; ++j)
+        {
+          const double *kp = S.kprev[j] + base;
+          const double fj = S.factor[j];
+          double t[2 * ND];
+#pragma unroll
+          for (int i = 0; i < 2 * ND; ++i)
+            t[i] = kp[(size_t)i * CB];
+          issue_barrier();
+#pragma unroll
+          for (int i = 0; i < 2 * ND; ++i)
+            acc[i] += fj * t[i];
+        }
+      const double fs = S.factor[S.stage];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int k = 0; k < ND; ++k)
-          {
-            const size_t idx = vix(batch, 2, c, ND, k, lane);
-            const double kk = double(Racc[c][k]);
-            double acc = S.wsol[idx] + S.factor[0] * (S.stage == 0 ? kk : S.kprev[0][idx]);
-            for (int j = 1; j <= S.stage; ++j)
-              acc += S.factor[j] * (j == S.stage ? kk : S.kprev[j][idx]);
-            S.rk_dst[idx] = acc;
-          }
+          S.rk_dst[base + (size_t)(c * ND + k) * CB] = acc[c * ND + k] + fs * double(Racc[c][k]);
     }
 }
 
@@ -690,11 +747,12 @@ template <int P, bool COUNT>
 __device__ __forceinline__ int depth_mass(const OpArgs &O, const DegreeTables &T,
                                           const Phys<double> &M, int batch, int lane,
                                           const double *Z, double *R, double sgn, double area,
-                                          unsigned long long &drybits)
+                                          unsigned long long &drybits, double &hmin)
 {
   constexpr int N1 = P + 1, NG = P + 2;
   const size_t zbase = (size_t)batch * NG * NG * CB + lane;
   int ndry = 0;
+  hmin = DBL_MAX;
 #pragma unroll 1
   for (int b = 0; b < NG; ++b)
     {
@@ -718,10 +776,14 @@ __device__ __forceinline__ int depth_mass(const OpArgs &O, const DegreeTables &T
             z += T.Sg[a][i] * zs[i];
           const int q = a + NG * b;
           const double zb = fld(O.cell1, zbase + (size_t)q * CB);
-          if (COUNT && z <= -zb)
+          if (COUNT)
             {
-              ++ndry;
-              drybits |= 1ull << q;
+              if (z <= -zb)
+                {
+                  ++ndry;
+                  drybits |= 1ull << q;
+                }
+              hmin = fmin(hmin, z + zb);
             }
           const double v = sgn * M.depth(z, zb) * (T.wg[a] * T.wg[b] * area);
 #pragma unroll
@@ -783,62 +845,115 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
       for (int i = 0; i < ND; ++i)
         Z[i] = -zbmax + 1e-09;
   }
-  double norm = 1.0;
+  // inverse lumped mass of a wet cell: 1/(l_i l_j) * (1/area) instead of 1/(l_i l_j area),
+  // a rounding-level difference that saves one FP64 division per DoF and sweep.
+  //
+  // Sweeps in the linear regime.  While every GL point stays wetter than the depth floor
+  // (z + zb > 1e-12: no dry point, floor not binding) the residual F - M h(Z) is affine in
+  // Z with the exact tensor-product mass area (M1 x M1), so after an exact sweep the next
+  // residuals follow from r <- r - area (M1 x M1) dZ (2 (p+1)^3 FMAs instead of two passes
+  // over the (p+2)^2 points).  The regime is certified per cell: |z_k - z_b| <= lam C at
+  // every GL point, C the l1 sum of the updates since the last exact sweep b, so
+  // hmin_b - lam C > 2e-12 keeps every point strictly wet; otherwise the sweep is exact.
+  const double inv_area = 1.0 / area;
+  double norm = 1.0, hbase = -1.0, C = 0.0;
+  bool linear = false;
+  double Rr[ND], dZ[ND];
   int k = 0;
   for (; k < O.max_it && norm > 1e-12; ++k)
     {
-      double Rr[ND];
-#pragma unroll
-      for (int i = 0; i < ND; ++i)
-        Rr[i] = F[i];
-      unsigned long long bits = 0;
-      const int ndry = depth_mass<P, true>(O, T, M, batch, lane, Z, Rr, -1.0, area, bits);
       double D[ND];
-      if (ndry == 0)
+      int ndry = 0;
+      if (linear && hbase - T.lam * C > 2e-12)
         {
+          double X[ND];
+#pragma unroll
+          for (int l = 0; l < N1; ++l)
+#pragma unroll
+            for (int i = 0; i < N1; ++i)
+              {
+                double s = 0;
+#pragma unroll
+                for (int m = 0; m < N1; ++m)
+                  s += T.M1[i][m] * dZ[m + N1 * l];
+                X[i + N1 * l] = s;
+              }
 #pragma unroll
           for (int j = 0; j < N1; ++j)
 #pragma unroll
             for (int i = 0; i < N1; ++i)
-              D[i + N1 * j] = T.lumped[i] * T.lumped[j] * area;
+              {
+                double s = 0;
+#pragma unroll
+                for (int l = 0; l < N1; ++l)
+                  s += T.M1[j][l] * X[i + N1 * l];
+                Rr[i + N1 * j] -= area * s;
+                D[i + N1 * j] = T.ilumped2[i][j] * inv_area;
+              }
         }
       else
         {
 #pragma unroll
           for (int i = 0; i < ND; ++i)
-            D[i] = 0;
-#pragma unroll 1
-          for (int b = 0; b < NG; ++b)
+            Rr[i] = F[i];
+          unsigned long long bits = 0;
+          double hmin;
+          ndry = depth_mass<P, true>(O, T, M, batch, lane, Z, Rr, -1.0, area, bits, hmin);
+          linear = (ndry == 0);
+          hbase = hmin;
+          C = 0.0;
+          if (ndry == 0)
             {
-              double W[N1];
-#pragma unroll
-              for (int i = 0; i < N1; ++i)
-                W[i] = 0;
-#pragma unroll
-              for (int a = 0; a < NG; ++a)
-                {
-                  const double mask = ((bits >> (a + NG * b)) & 1ull) ? 0.0 : 1.0;
-                  const double v = mask * (T.wg[a] * T.wg[b] * area);
-#pragma unroll
-                  for (int i = 0; i < N1; ++i)
-                    W[i] += T.Sg[a][i] * v;
-                }
 #pragma unroll
               for (int j = 0; j < N1; ++j)
 #pragma unroll
                 for (int i = 0; i < N1; ++i)
-                  D[i + N1 * j] += T.Sg[b][j] * W[i];
+                  D[i + N1 * j] = T.ilumped2[i][j] * inv_area;
+            }
+          else
+            {
+#pragma unroll
+              for (int i = 0; i < ND; ++i)
+                D[i] = 0;
+#pragma unroll 1
+              for (int b = 0; b < NG; ++b)
+                {
+                  double W[N1];
+#pragma unroll
+                  for (int i = 0; i < N1; ++i)
+                    W[i] = 0;
+#pragma unroll
+                  for (int a = 0; a < NG; ++a)
+                    {
+                      const double mask = ((bits >> (a + NG * b)) & 1ull) ? 0.0 : 1.0;
+                      const double v = mask * (T.wg[a] * T.wg[b] * area);
+#pragma unroll
+                      for (int i = 0; i < N1; ++i)
+                        W[i] += T.Sg[a][i] * v;
+                    }
+#pragma unroll
+                  for (int j = 0; j < N1; ++j)
+#pragma unroll
+                    for (int i = 0; i < N1; ++i)
+                      D[i + N1 * j] += T.Sg[b][j] * W[i];
+                }
+#pragma unroll
+              for (int i = 0; i < ND; ++i)
+                D[i] = 1. / D[i];
             }
         }
+      // D holds the inverse lumped mass here
       norm = 0.0;
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         {
-          const double dz = (ndry < NP) ? (1. / D[i]) * Rr[i] : 0.0;
+          const double dz = (ndry < NP) ? D[i] * Rr[i] : 0.0;
           const double old = Z[i];
           Z[i] += dz;
-          norm += fabs(old - Z[i]);
+          dZ[i] = Z[i] - old;
+          norm += fabs(dZ[i]);
         }
+      C += norm;
     }
   return k;
 }
@@ -918,7 +1033,8 @@ __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTa
   for (int i = 0; i < ND; ++i)
     F[i] = r[i];
   unsigned long long bits = 0;
-  depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
+  double hdummy;
+  depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits, hdummy);
   return jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
 }
 
@@ -952,22 +1068,28 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
       const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
       const double area = hx * hy;
       double F[ND], Z[ND];
+      const size_t base = (size_t)batch * ND * CB + lane;
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         {
-          const size_t idx = vix(batch, 1, 0, ND, i, lane);
-          double f = S.factor[0] * S.kprev[0][idx];
-          for (int j = 1; j <= S.stage; ++j)
-            f += S.factor[j] * S.kprev[j][idx];
-          F[i] = f;
-          Z[i] = S.wsol[idx];
+          F[i] = S.factor[0] * S.kprev[0][base + (size_t)i * CB];
+          Z[i] = S.wsol[base + (size_t)i * CB];
+        }
+      for (int j = 1; j <= S.stage; ++j)
+        {
+          const double *kp = S.kprev[j] + base;
+          const double fj = S.factor[j];
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            F[i] += fj * kp[(size_t)i * CB];
         }
       if (centred<R>())
         k = mixed_depth_solve<P>(O, T, M, batch, lane, area, F, Z);
       else
         {
           unsigned long long bits = 0;
-          depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
+          double hdummy;
+          depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits, hdummy);
           if (S.scratch)
 #pragma unroll
             for (int i = 0; i < ND; ++i)
@@ -1006,16 +1128,44 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
         height_operator<P, R>(O, basis<R>(P), physics<R>(O), batch, lane, R(hx), R(hy),
                               (const R *)Zo, (const R *)QX, (const R *)QY, zc, S.src_h,
                               S.src_q, Racc);
+        // front sum_j a_j k_{j+1} in the reference's order, the earlier k vectors read one
+        // whole vector at a time (loads in flight together)
+        const size_t base = (size_t)batch * ND * CB + lane;
 #pragma unroll
         for (int i = 0; i < ND; ++i)
+          S.dst[base + (size_t)i * CB] = double(Racc[0][i]);  // ki_h[stage+1]
+        if (S.stage == 0)
           {
-            const size_t idx = vix(batch, 1, 0, ND, i, lane);
-            const double kk = double(Racc[0][i]);
-            S.dst[idx] = kk;  // ki_h[stage+1]
-            double f = S.factor[0] * (S.stage == 0 ? kk : S.kprev[0][idx]);
-            for (int j = 1; j <= S.stage; ++j)
-              f += S.factor[j] * (j == S.stage ? kk : S.kprev[j][idx]);
-            F[i] = f;
+#pragma unroll
+            for (int i = 0; i < ND; ++i)
+              F[i] = S.factor[0] * double(Racc[0][i]);
+          }
+        else
+          {
+#pragma unroll
+            for (int i = 0; i < ND; ++i)
+              F[i] = S.kprev[0][base + (size_t)i * CB];
+            issue_barrier();
+#pragma unroll
+            for (int i = 0; i < ND; ++i)
+              F[i] = S.factor[0] * F[i];
+            for (int j = 1; j < S.stage; ++j)
+              {
+                const double *kp = S.kprev[j] + base;
+                const double fj = S.factor[j];
+                double t[ND];
+#pragma unroll
+                for (int i = 0; i < ND; ++i)
+                  t[i] = kp[(size_t)i * CB];
+                issue_barrier();
+#pragma unroll
+                for (int i = 0; i < ND; ++i)
+                  F[i] += fj * t[i];
+              }
+            const double fs = S.factor[S.stage];
+#pragma unroll
+            for (int i = 0; i < ND; ++i)
+              F[i] += fs * double(Racc[0][i]);
           }
       }
       phase_barrier();
@@ -1023,12 +1173,14 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         Z[i] = S.wsol[vix(batch, 1, 0, ND, i, lane)];
+      issue_barrier();
       if (centred<R>())
         k = mixed_depth_solve<P>(O, T, M, batch, lane, area, F, Z);
       else
         {
           unsigned long long bits = 0;
-          depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits);
+          double hdummy;
+          depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits, hdummy);
           if (S.scratch)
 #pragma unroll
             for (int i = 0; i < ND; ++i)

@@ -66,8 +66,9 @@ int main()
   oceano::check(oc_dg_project_collocation(op.handle(), 1, zv.data(), zc.data()), "project");
   oceano::check(oc_dg_project_collocation(op.handle(), 2, qv.data(), qc.data()), "project");
   DeviceVector tz(zc.size()), tq(qc.size());
-  hipMemcpy(tz.get_values(), zc.data(), zc.size() * 8, hipMemcpyHostToDevice);
-  hipMemcpy(tq.get_values(), qc.data(), qc.size() * 8, hipMemcpyHostToDevice);
+  if (hipMemcpy(tz.get_values(), zc.data(), zc.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(tq.get_values(), qc.data(), qc.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    throw std::runtime_error("copy");
   DeviceVector h0, q0;
   op.initialize_vector(h0, 0);
   op.initialize_vector(q0, 1);
