@@ -43,7 +43,8 @@ const char *oc_last_error(void);
  *   "slot_chunk"  cells are grouped by face signature (which faces are the coarse side of
  *                 two subfaces or on the boundary) within chunks of this many 64-cell
  *                 batches of the caller's order; 0 (default) = one XCD's share of the
- *                 batches, < 0 = keep the caller's order.  Results do not depend on it. */
+ *                 batches for OC_PRECISION_FP64 and the caller's order for the mixed
+ *                 path; < 0 = keep the caller's order.  Results do not depend on it. */
 int oc_set_option(const char *name, int64_t value);
 
 /* ------------------------------------------------------------------------------------

@@ -366,9 +366,13 @@ bool uniform(const std::vector<double> &v, double &val)
 // that almost every batch of 64 has one signature and runs one face path.
 // Sorting is done within chunks of consecutive cells of the caller's order: a chunk of one
 // XCD's share of the batches keeps each XCD's neighbour traces in its own L2.
-int g_slot_chunk = 0;  // batches per chunk; 0: one XCD's share; < 0: the caller's order
+// Default (0): FP64 sorts within one XCD's share (fewer divergent face passes pay off at one
+// wave per SIMD); the mixed path keeps the caller's order (its two waves per SIMD hide the
+// divergence and it gains more from neighbour locality) -- measured on the C2 vortex.
+int g_slot_chunk = 0;  // batches per chunk; 0: by precision as above; < 0: the caller's order
 
-int cell_slots(const oc_mesh_desc *mesh, std::vector<int32_t> &order, std::vector<int32_t> &pos)
+int cell_slots(const oc_mesh_desc *mesh, int precision, std::vector<int32_t> &order,
+               std::vector<int32_t> &pos)
 {
   const int N = mesh->n_cells;
   std::vector<int> code((size_t)N * 4, 0);  // 0 conforming / fine side, 1 coarse, 2 boundary
@@ -401,8 +405,9 @@ int cell_slots(const oc_mesh_desc *mesh, std::vector<int32_t> &order, std::vecto
   for (int c = 0; c < N; ++c)
     order[c] = c;
   const int nb = (N + CB - 1) / CB;
-  const long chunk = g_slot_chunk < 0 ? 0
-                     : (long)CB * (g_slot_chunk > 0 ? g_slot_chunk : (nb + NXCD - 1) / NXCD);
+  const bool sort = g_slot_chunk > 0 || (g_slot_chunk == 0 && precision == OC_PRECISION_FP64);
+  const long chunk =
+    !sort ? 0 : (long)CB * (g_slot_chunk > 0 ? g_slot_chunk : (nb + NXCD - 1) / NXCD);
   for (long c0 = 0; chunk > 0 && c0 < N; c0 += chunk)
     std::stable_sort(order.begin() + c0, order.begin() + std::min<long>(N, c0 + chunk),
                      [&](int a, int b) { return sig[a] < sig[b]; });
@@ -489,7 +494,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       op->bc_of_id[prm->boundary_ids[i]] = kind;
     }
   const int N = op->N, NB = op->NB;
-  if (const int rc2 = cell_slots(mesh, op->order, op->pos))
+  if (const int rc2 = cell_slots(mesh, op->precision, op->order, op->pos))
     {
       delete op;
       return rc2;

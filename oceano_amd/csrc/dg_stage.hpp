@@ -120,12 +120,27 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P), NG = P + 2;
   const R zcr = R(zc);
+  // the face tables of all four faces in one group of loads: each subface then waits for
+  // one memory latency (its neighbour's traces) instead of three dependent ones
+  int infos[4], nbrs[4][2], fids[4][2];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+    {
+      const size_t fo = ((size_t)batch * 4 + f);
+      infos[f] = O.finfo[fo * CB + lane];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        {
+          nbrs[f][s] = O.nbr[(fo * 2 + s) * CB + lane];
+          fids[f][s] = O.fid[(fo * 2 + s) * CB + lane];
+        }
+    }
+  issue_barrier();
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     {
       phase_barrier();
-      const size_t fo = ((size_t)batch * 4 + f);
-      const int info = O.finfo[fo * CB + lane];
+      const int info = infos[f];
       const int kind = info & 3;
       const int nsub = (info >> 2) & 1;
       const R flen = (f < 2) ? hy : hx;
@@ -139,7 +154,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
         }
       if (kind == FK_BDRY)
         {
-          const int bf = O.fid[(fo * 2) * CB + lane];
+          const int bf = fids[f][0];
           const int bk = O.bkind[bf];
           const Vec2<R> n = {f == 0 ? R(-1) : f == 1 ? R(1) : R(0),
                              f == 2 ? R(-1) : f == 3 ? R(1) : R(0)};
@@ -174,35 +189,42 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
         {
           const bool is_m = (f & 1);  // canonical orientation: minus side owns faces 1, 3
           const Vec2<R> n = {f < 2 ? R(1) : R(0), f < 2 ? R(0) : R(1)};
+          // neighbour traces of both subfaces in one group of loads (the second only on
+          // the coarse side): one memory latency per face
+          phase_barrier();
+          double zg[2][N1], xg[2][N1], yg[2][N1], zc2[2];
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+            if (s == 0 || kind == FK_COARSE)
+              {
+                const int b2 = nbrs[f][s] / CB, l2 = nbrs[f][s] % CB;
+                zc2[s] = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
+#pragma unroll
+                for (int k = 0; k < N1; ++k)
+                  {
+                    const int d = face_dof<N1>(f ^ 1, k);
+                    zg[s][k] = src_h[vix(b2, 1, 0, ND, d, l2)];
+                    xg[s][k] = src_q[vix(b2, 2, 0, ND, d, l2)];
+                    yg[s][k] = src_q[vix(b2, 2, 1, ND, d, l2)];
+                  }
+              }
+          issue_barrier();
 #pragma unroll
           for (int s = 0; s < 2; ++s)
             {
               if (s == 1 && kind != FK_COARSE)
                 break;
               phase_barrier();
-              const int c2 = O.nbr[(fo * 2 + s) * CB + lane];
-              const int fi = O.fid[(fo * 2 + s) * CB + lane];
-              const int b2 = c2 / CB, l2 = c2 % CB;
-              double zg[N1], xg[N1], yg[N1];
-              const double zc2 = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
-#pragma unroll
-              for (int k = 0; k < N1; ++k)
-                {
-                  const int d = face_dof<N1>(f ^ 1, k);
-                  zg[k] = src_h[vix(b2, 1, 0, ND, d, l2)];
-                  xg[k] = src_q[vix(b2, 2, 0, ND, d, l2)];
-                  yg[k] = src_q[vix(b2, 2, 1, ND, d, l2)];
-                }
-              issue_barrier();
+              const int fi = fids[f][s];
               R zn_l[N1], xn_l[N1], yn_l[N1];
 #pragma unroll
               for (int k = 0; k < N1; ++k)
                 {
-                  zn_l[k] = R(zg[k] - zc2);
-                  xn_l[k] = R(xg[k]);
-                  yn_l[k] = R(yg[k]);
+                  zn_l[k] = R(zg[s][k] - zc2[s]);
+                  xn_l[k] = R(xg[s][k]);
+                  yn_l[k] = R(yg[s][k]);
                 }
-              const R zc2r = R(zc2);
+              const R zc2r = R(zc2[s]);
               const R len = (kind == FK_COARSE) ? R(0.5) * flen : flen;
 #pragma unroll 1
               for (int a = 0; a < NG; ++a)
@@ -703,31 +725,38 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       // post-op of OceanDG.h:1878-1909: dst = w + sum_j a_j k_{j+1}, k_{stage+1} = Racc,
       // summed in the reference's order; the loop over the earlier k vectors is outside
       // the loop over the DoFs so that the 2 nd loads of one vector are in flight together
-      const size_t base = (size_t)batch * 2 * ND * CB + lane;
-      double acc[2 * ND];
+      // FP32 integrals: one component per pass keeps the kernel within 3 waves per SIMD
+      constexpr int NCH = std::is_same<R, float>::value ? 2 : 1, CH = 2 * ND / NCH;
 #pragma unroll
-      for (int i = 0; i < 2 * ND; ++i)
-        acc[i] = S.wsol[base + (size_t)i * CB];
-      issue_barrier();
-      for (int j = 0; j < S.stage; ++j)
+      for (int ch = 0; ch < NCH; ++ch)
         {
-          const double *kp = S.kprev[j] + base;
-          const double fj = S.factor[j];
-          double t[2 * ND];
+          const size_t base = ((size_t)batch * 2 * ND + ch * CH) * CB + lane;
+          double acc[CH];
 #pragma unroll
-          for (int i = 0; i < 2 * ND; ++i)
-            t[i] = kp[(size_t)i * CB];
+          for (int i = 0; i < CH; ++i)
+            acc[i] = S.wsol[base + (size_t)i * CB];
           issue_barrier();
+          for (int j = 0; j < S.stage; ++j)
+            {
+              const double *kp = S.kprev[j] + base;
+              const double fj = S.factor[j];
+              double t[CH];
 #pragma unroll
-          for (int i = 0; i < 2 * ND; ++i)
-            acc[i] += fj * t[i];
+              for (int i = 0; i < CH; ++i)
+                t[i] = kp[(size_t)i * CB];
+              issue_barrier();
+#pragma unroll
+              for (int i = 0; i < CH; ++i)
+                acc[i] += fj * t[i];
+            }
+          const double fs = S.factor[S.stage];
+#pragma unroll
+          for (int i = 0; i < CH; ++i)
+            {
+              const int c = (ch * CH + i) / ND, k = (ch * CH + i) % ND;
+              S.rk_dst[base + (size_t)i * CB] = acc[i] + fs * double(Racc[c][k]);
+            }
         }
-      const double fs = S.factor[S.stage];
-#pragma unroll
-      for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int k = 0; k < ND; ++k)
-          S.rk_dst[base + (size_t)(c * ND + k) * CB] = acc[c * ND + k] + fs * double(Racc[c][k]);
     }
 }
 
@@ -858,102 +887,107 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
   const double inv_area = 1.0 / area;
   double norm = 1.0, hbase = -1.0, C = 0.0;
   bool linear = false;
-  double Rr[ND], dZ[ND];
+  double Rr[ND];
+  // Z += Dinv Rr and its l1 norm; the residual is advanced to the new Z by the linear
+  // formula (used only if the next sweep is certified linear, recomputed otherwise)
+  auto update = [&](auto dinv) {
+    norm = 0.0;
+    double dZ[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+      {
+        const double old = Z[i];
+        Z[i] += dinv(i) * Rr[i];
+        dZ[i] = Z[i] - old;
+        norm += fabs(dZ[i]);
+      }
+    C += norm;
+#pragma unroll
+    for (int l = 0; l < N1; ++l)
+      {
+        double x[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i)
+          {
+            double s = 0;
+#pragma unroll
+            for (int m = 0; m < N1; ++m)
+              s += T.M1[i][m] * dZ[m + N1 * l];
+            x[i] = s;
+          }
+#pragma unroll
+        for (int i = 0; i < N1; ++i)
+          dZ[i + N1 * l] = x[i];  // dZ now holds (M1 x I) dZ
+      }
+#pragma unroll
+    for (int j = 0; j < N1; ++j)
+#pragma unroll
+      for (int i = 0; i < N1; ++i)
+        {
+          double s = 0;
+#pragma unroll
+          for (int l = 0; l < N1; ++l)
+            s += T.M1[j][l] * dZ[i + N1 * l];
+          Rr[i + N1 * j] -= area * s;
+        }
+  };
+  auto wet_dinv = [&](int i) { return T.ilumped2[i % N1][i / N1] * inv_area; };
   int k = 0;
   for (; k < O.max_it && norm > 1e-12; ++k)
     {
-      double D[ND];
-      int ndry = 0;
       if (linear && hbase - T.lam * C > 2e-12)
         {
-          double X[ND];
-#pragma unroll
-          for (int l = 0; l < N1; ++l)
-#pragma unroll
-            for (int i = 0; i < N1; ++i)
-              {
-                double s = 0;
-#pragma unroll
-                for (int m = 0; m < N1; ++m)
-                  s += T.M1[i][m] * dZ[m + N1 * l];
-                X[i + N1 * l] = s;
-              }
-#pragma unroll
-          for (int j = 0; j < N1; ++j)
-#pragma unroll
-            for (int i = 0; i < N1; ++i)
-              {
-                double s = 0;
-#pragma unroll
-                for (int l = 0; l < N1; ++l)
-                  s += T.M1[j][l] * X[i + N1 * l];
-                Rr[i + N1 * j] -= area * s;
-                D[i + N1 * j] = T.ilumped2[i][j] * inv_area;
-              }
+          update(wet_dinv);
+          continue;
         }
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        Rr[i] = F[i];
+      unsigned long long bits = 0;
+      double hmin;
+      const int ndry = depth_mass<P, true>(O, T, M, batch, lane, Z, Rr, -1.0, area, bits, hmin);
+      linear = (ndry == 0);
+      hbase = hmin;
+      C = 0.0;
+      if (ndry == 0)
+        update(wet_dinv);
+      else if (ndry == NP)
+        update([](int) { return 0.0; });
       else
         {
+          // lumped mass of the wet part (OceanDG.h:1470-1489); 1/D_i = inf where no wet
+          // point supports phi_i, as in the reference
+          double D[ND];
 #pragma unroll
           for (int i = 0; i < ND; ++i)
-            Rr[i] = F[i];
-          unsigned long long bits = 0;
-          double hmin;
-          ndry = depth_mass<P, true>(O, T, M, batch, lane, Z, Rr, -1.0, area, bits, hmin);
-          linear = (ndry == 0);
-          hbase = hmin;
-          C = 0.0;
-          if (ndry == 0)
+            D[i] = 0;
+#pragma unroll 1
+          for (int b = 0; b < NG; ++b)
             {
+              double W[N1];
+#pragma unroll
+              for (int i = 0; i < N1; ++i)
+                W[i] = 0;
+#pragma unroll
+              for (int a = 0; a < NG; ++a)
+                {
+                  const double mask = ((bits >> (a + NG * b)) & 1ull) ? 0.0 : 1.0;
+                  const double v = mask * (T.wg[a] * T.wg[b] * area);
+#pragma unroll
+                  for (int i = 0; i < N1; ++i)
+                    W[i] += T.Sg[a][i] * v;
+                }
 #pragma unroll
               for (int j = 0; j < N1; ++j)
 #pragma unroll
                 for (int i = 0; i < N1; ++i)
-                  D[i + N1 * j] = T.ilumped2[i][j] * inv_area;
+                  D[i + N1 * j] += T.Sg[b][j] * W[i];
             }
-          else
-            {
 #pragma unroll
-              for (int i = 0; i < ND; ++i)
-                D[i] = 0;
-#pragma unroll 1
-              for (int b = 0; b < NG; ++b)
-                {
-                  double W[N1];
-#pragma unroll
-                  for (int i = 0; i < N1; ++i)
-                    W[i] = 0;
-#pragma unroll
-                  for (int a = 0; a < NG; ++a)
-                    {
-                      const double mask = ((bits >> (a + NG * b)) & 1ull) ? 0.0 : 1.0;
-                      const double v = mask * (T.wg[a] * T.wg[b] * area);
-#pragma unroll
-                      for (int i = 0; i < N1; ++i)
-                        W[i] += T.Sg[a][i] * v;
-                    }
-#pragma unroll
-                  for (int j = 0; j < N1; ++j)
-#pragma unroll
-                    for (int i = 0; i < N1; ++i)
-                      D[i + N1 * j] += T.Sg[b][j] * W[i];
-                }
-#pragma unroll
-              for (int i = 0; i < ND; ++i)
-                D[i] = 1. / D[i];
-            }
+          for (int i = 0; i < ND; ++i)
+            D[i] = 1. / D[i];
+          update([&](int i) { return D[i]; });
         }
-      // D holds the inverse lumped mass here
-      norm = 0.0;
-#pragma unroll
-      for (int i = 0; i < ND; ++i)
-        {
-          const double dz = (ndry < NP) ? D[i] * Rr[i] : 0.0;
-          const double old = Z[i];
-          Z[i] += dz;
-          dZ[i] = Z[i] - old;
-          norm += fabs(dZ[i]);
-        }
-      C += norm;
     }
   return k;
 }
