@@ -70,6 +70,11 @@ typedef struct oc_mesh oc_mesh;
 int oc_mesh_create_rectangle(double x0, double x1, double y0, double y1, int32_t nx,
                              int32_t ny, int32_t periodic_x, int32_t periodic_y,
                              const int32_t boundary_id[4], oc_mesh **mesh);
+/* Gmsh 4.1 ASCII file (GridIn::read_msh, main.cc:621-637) whose quadrilaterals tile a
+ * uniform axis-aligned grid, e.g. the reference decks' shallowWaterVortex_20x10cells.msh;
+ * boundary ids from the physical tags of the boundary curves, one per side. */
+int oc_mesh_read_msh(const char *path, int32_t periodic_x, int32_t periodic_y,
+                     oc_mesh **mesh);
 /* split every active cell into four (Triangulation::refine_global) */
 int oc_mesh_refine_global(oc_mesh *mesh, int32_t times);
 /* split round(fraction * n_active) active cells chosen by a Fisher-Yates shuffle driven
@@ -204,6 +209,50 @@ int oc_dg_project_collocation(const oc_dg *op, int32_t n_comp, const double *val
  * Depths use the z_b of OC_STORE_CELL_GAUSS.  Host memory, no GPU work. */
 int oc_dg_compute_errors(const oc_dg *op, const double *zeta, const double *q,
                          const double *exact, double err[7]);
+
+/* ------------------------------------------------------------------------------------
+ * Work around the stage that the time loop needs (main.cc:1546-1577), on the GPU.
+ *
+ * Velocity field for the viscous term and output: evaluate_velocity_field
+ * (OceanDG.h:2600-2662) -- velocity at the Gauss(p+1) points (z_b of
+ * OC_STORE_CELL_COLLOC), collocated back to the nodal basis.  vel: operator order, 2
+ * components.  Asynchronous on `stream`. */
+int oc_dg_velocity(oc_dg *op, const double *zeta, const double *q, double *vel, void *stream);
+
+/* Largest transport speed over the mesh: compute_cell_transport_speed
+ * (OceanDG.h:2506-2580), which the time loop turns into the step every five steps
+ * (main.cc:1550-1555).  Synchronises the stream; *speed on the host. */
+int oc_dg_transport_speed(oc_dg *op, const double *zeta, const double *q, double *speed,
+                          void *stream);
+
+/* Mass-conservation check (OceanDG.h:2138-2184): *volume = integral of the depth
+ * max(zeta + z_b, 1e-12) at the GL(p+2) points (z_b of OC_STORE_CELL_LOBATTO); *sum =
+ * sum of all entries of an operator-order vector (the boundary part of the check is the
+ * sum of ki_h[s+1] weighted by the step's b_s dt: inner faces and cell terms of L_zeta sum
+ * to zero).  Both synchronise the stream. */
+int oc_dg_volume(oc_dg *op, const double *zeta, double *volume, void *stream);
+int oc_dg_sum(oc_dg *op, const double *x, int32_t n_components, double *sum, void *stream);
+
+/* ------------------------------------------------------------------------------------
+ * Device-resident explicit RK time loop (ExplicitRungeKuttaIntegrator.h:179-298 driven as
+ * in main.cc:1546-1577).  The integrator owns S+1 k registers per field and two r
+ * registers that alternate between stages (the reference copies r into vec_ri instead;
+ * the results are the same).  rows: S rows packed one after the other, row s holding
+ * s+1 coefficients -- the Butcher row a[s+1][0..s] for s < S-1 and b for s = S-1
+ * (unscaled; the step multiplies by dt).  A step is the S stages, then, when `velocity`
+ * is given (viscosity != 0), the velocity update of the new state.  oc_rk_capture records
+ * one step with fixed arguments into a HIP graph (on a non-null stream); oc_rk_replay
+ * launches it n times.  Boundary data are whatever was tabulated before the capture. */
+typedef struct oc_rk oc_rk;
+int oc_rk_create(oc_dg *op, int32_t n_stages, const double *rows, oc_rk **out);
+void oc_rk_destroy(oc_rk *rk);
+int oc_rk_step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity, void *stream);
+int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity,
+                  void *stream);
+int oc_rk_replay(oc_rk *rk, int32_t n, void *stream);
+/* device pointers of the integrator's registers (arrays of S+1 and 2 entries; any may be
+ * null) */
+int oc_rk_registers(oc_rk *rk, double **ki_h, double **ki_q, double **r_h, double **r_q);
 
 /* Count non-finite entries of a device vector of `n` doubles into *count (host).  The
  * reference has no such check (main.cc:1527-1529 only mentions blow-up); hosts use it

@@ -63,7 +63,24 @@ class Params(C.Structure):
 SIGNATURES = {
     "oc_last_error": (C.c_char_p, []),
     "oc_set_option": (C.c_int, [C.c_char_p, C.c_int64]),
+    "oc_dg_velocity": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "oc_dg_transport_speed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.POINTER(C.c_double), C.c_void_p]),
+    "oc_dg_volume": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "oc_dg_sum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                            C.c_void_p]),
+    "oc_rk_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                               C.POINTER(C.c_void_p)]),
+    "oc_rk_destroy": (None, [C.c_void_p]),
+    "oc_rk_step": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p]),
+    "oc_rk_capture": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p]),
+    "oc_rk_replay": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "oc_rk_registers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                  C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "oc_mesh_create_rectangle": (C.c_int, [D, D, D, D, I32, I32, I32, I32, PI32, C.POINTER(P)]),
+    "oc_mesh_read_msh": (C.c_int, [C.c_char_p, I32, I32, C.POINTER(P)]),
     "oc_mesh_refine_global": (C.c_int, [P, I32]),
     "oc_mesh_refine_fraction": (C.c_int, [P, D, U64]),
     "oc_mesh_info": (C.c_int, [P, PI64]),

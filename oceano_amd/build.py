@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liboceano_dg.so")
 
-SOURCES = ["mesh.cpp", "icbc.cpp", "dg_operator.hip",
+SOURCES = ["mesh.cpp", "icbc.cpp", "dg_operator.hip", "dg_rk.hip",
            "dg_stage_p1.hip", "dg_stage_p2.hip", "dg_stage_p3.hip", "dg_stage_p4.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -40,6 +40,8 @@ struct oc_dg
   double c_cell0[5] = {};
   double *d_cell1 = nullptr;
   double c_cell1 = 0;
+  double *d_cell2 = nullptr;
+  double c_cell2 = 0;
   double *d_face = nullptr;
   double c_face = 0;
   double *d_bz = nullptr;
@@ -47,6 +49,8 @@ struct oc_dg
   double *d_bvals = nullptr;
   unsigned int *d_sweeps_blk = nullptr;
   float *d_jmat = nullptr;  // mixed precision: closed-form Jacobi matrix (unit cell)
+  int *d_defer = nullptr;   // mixed precision: cells whose FP64 depth solve is deferred
+  double *d_post = nullptr; // per-batch results of the reductions [NB]
   double wet_margin = 10.0;
   unsigned long long *d_count = nullptr;
   std::vector<double> cell2_host;
@@ -69,6 +73,7 @@ struct oc_dg
     for (int f = 0; f < 5; ++f)
       o.cell0[f] = {d_cell0[f], c_cell0[f]};
     o.cell1 = {d_cell1, c_cell1};
+    o.cell2 = {d_cell2, c_cell2};
     o.facez = {d_face, c_face};
     o.bdryz = {d_bz, c_bz};
     o.bvals = d_bvals;
@@ -84,6 +89,7 @@ struct oc_dg
     o.xcd_swizzle = xcd_swizzle;
     o.jmat = d_jmat;
     o.wet_margin = wet_margin;
+    o.defer = d_defer;
     return o;
   }
 };
@@ -112,7 +118,16 @@ namespace oceano {
                  double *w_q, double *r_h, double *r_q, int write_scratch, hipStream_t s);  \
   int rhs_apply_d##P(oc_dg *op, const OpArgs &O, bool invert, const double *src_h,          \
                      const double *src_q, const double *src_v, double *dst_h,              \
-                     double *dst_q, hipStream_t s);
+                     double *dst_q, hipStream_t s);                                        \
+  int post_d##P(oc_dg *op, const OpArgs &O, int what, const double *zeta, const double *q,  \
+                double *out, hipStream_t s);
+// per-degree post-processing kernels (post_dP)
+enum PostKernel
+{
+  POST_VELOCITY = 0,  // out: velocity field, operator order, 2 components
+  POST_SPEED = 1,     // out: per-batch maximum transport speed [NB]
+  POST_VOLUME = 2     // out: per-batch integral of the depth [NB]
+};
 // kernels timed by oc_dg_set_timing
 enum TimedKernel
 {

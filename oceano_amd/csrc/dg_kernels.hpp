@@ -36,6 +36,7 @@ struct Basis
   R lumped[MAXN1];         // sum_a Sg[a][i] wg[a]
   R ilumped2[MAXN1][MAXN1];  // 1 / (lumped[i] lumped[j]): inverse lumped mass / area
   R M1[MAXN1][MAXN1];        // sum_a Sg[a][i] Sg[a][j] wg[a]: 1-D GL(ng) mass (exact)
+  R Sn[MAXN1][MAXN1];        // l_i at Gauss(n1) points (velocity projection)
   R lam;                     // max_{a,i} Sg[a][i]^2: bound of |phi_ij| at the GL(ng)^2 points
 };
 using DegreeTables = Basis<double>;
@@ -64,6 +65,7 @@ struct OpArgs
   const int *finfo;     // [batch][4][lane]    kind | subface << 2
   Field cell0[5];       // [batch][nq^2][lane] z_b, manning, tau_x, tau_y, coriolis
   Field cell1;          // [batch][ng^2][lane] z_b at GL points
+  Field cell2;          // [batch][n1^2][lane] z_b at Gauss(n1) points
   Field facez;          // [face][ng][2]
   Field bdryz;          // [bface][nq]
   const double *bvals;  // [bface][nq][3]
@@ -76,6 +78,7 @@ struct OpArgs
   // [nd][nd] row-major (delta = J r / area), and the wet-cell safety factor
   const float *jmat;
   double wet_margin;
+  int *defer;  // mixed path: per cell slot, 1 = depth solve left to k_depth_deferred
 };
 
 struct StageArgs

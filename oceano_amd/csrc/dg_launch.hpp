@@ -2,6 +2,7 @@
 #pragma once
 
 #include "dg_internal.hpp"
+#include "dg_post.hpp"
 #include "dg_stage.hpp"
 
 namespace oceano {
@@ -83,6 +84,19 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
         hipLaunchKernelGGL((k_stage<P, false, R>), dim3(2 * op->NB), block, 0, s, O, H, Q);
       timed_end(op, s, TK_STAGE, ev);
       HIP_TRY(hipGetLastError());
+      if (std::is_same<R, float>::value)
+        {
+          // FP64 depth solve of the cells the FP32 stage left near the depth floor; a wave
+          // without such a cell returns at once
+          StageArgs D = H;
+          for (uint32_t j = 0; j <= stage; ++j)
+            D.kprev[j] = ki_h[j + 1];
+          timed_begin(op, s, &ev);
+          const dim3 dgrid((op->NB + DEFER_BATCHES - 1) / DEFER_BATCHES);
+          hipLaunchKernelGGL((k_depth_deferred<P>), dgrid, block, 0, s, O, D);
+          timed_end(op, s, TK_HEIGHT_JACOBI, ev);
+          HIP_TRY(hipGetLastError());
+        }
       return OC_OK;
     }
   // split path: every kernel finishes reading ri before the next one overwrites it

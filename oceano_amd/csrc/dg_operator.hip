@@ -63,6 +63,9 @@ DegreeTables make_tables(int p)
       for (int i = 0; i < n1; ++i)
         S[a * n1 + i] = lagrange_value(nodes.x, i, rn.x[a]);
     }
+  for (int a = 0; a < n1; ++a)
+    for (int i = 0; i < n1; ++i)
+      t.Sn[a][i] = S[a * n1 + i];
   const std::vector<double> Si = small_inverse(S, n1);  // Si[i][a]
   for (int i = 0; i < n1; ++i)
     for (int a = 0; a < n1; ++a)
@@ -233,6 +236,22 @@ __global__ void k_pack(int n_cells, int ncomp, int nd, const int *__restrict__ o
 
 __global__ void k_zero_count(unsigned long long *p) { *p = 0ull; }
 
+// per batch: the sum of the active lanes' entries of an operator-order vector
+__global__ void k_batch_sum(int n_cells, int per_lane, const double *__restrict__ x,
+                            double *__restrict__ out_blk)
+{
+  const int batch = blockIdx.x, lane = threadIdx.x;
+  double s = 0.0;
+  if (batch * CB + lane < n_cells)
+    for (int k = 0; k < per_lane; ++k)
+      s += x[((size_t)batch * per_lane + k) * CB + lane];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    s += __shfl_xor(s, off);
+  if (lane == 0)
+    out_blk[batch] = s;
+}
+
 __global__ void k_count_nonfinite(const double *__restrict__ x, size_t n,
                                   unsigned long long *count)
 {
@@ -313,11 +332,14 @@ void free_all(oc_dg *op)
   for (auto p : op->d_cell0)
     f(p);
   f(op->d_cell1);
+  f(op->d_cell2);
   f(op->d_face);
   f(op->d_bz);
   f(op->d_bvals);
   f(op->d_sweeps_blk);
   f(op->d_jmat);
+  f(op->d_defer);
+  f(op->d_post);
   f(op->d_count);
 }
 
@@ -636,6 +658,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       (rc = upload(&op->d_fid, fid)) || (rc = upload(&op->d_finfo, finfo)) ||
       (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps_blk, (size_t)NB)) ||
       (rc = upload(&op->d_jmat, jacobi_matrix(op->P, op->max_it))) ||
+      (rc = dalloc(&op->d_defer, (size_t)NB * CB)) || (rc = dalloc(&op->d_post, (size_t)NB)) ||
       (rc = dalloc(&op->d_count, 1)))
     {
       free_all(op);
@@ -850,8 +873,13 @@ int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
           return interleave(cm, np, &op->d_cell1, op->c_cell1);
         }
       case OC_STORE_CELL_COLLOC:
-        op->cell2_host.assign(values, values + (size_t)N * op->nd * 2);
-        return OC_OK;
+        {
+          op->cell2_host.assign(values, values + (size_t)N * op->nd * 2);
+          std::vector<double> cm((size_t)N * op->nd);
+          for (size_t i = 0; i < cm.size(); ++i)
+            cm[i] = values[2 * i];
+          return interleave(cm, op->nd, &op->d_cell2, op->c_cell2);
+        }
       case OC_STORE_FACE:
         {
           const size_t ni = op->inner.size() / 6;
@@ -1174,6 +1202,93 @@ int oc_dg_timing_report(oc_dg *op, double *ms, int64_t *count)
       op->event_pool.push_back(e.second.second);
     }
   op->events.clear();
+  return OC_OK;
+}
+
+static int post_dispatch(oc_dg *op, int what, const double *zeta, const double *q, double *out,
+                         hipStream_t s)
+{
+  const OpArgs O = op->args();
+  switch (op->P)
+    {
+      case 1:
+        return post_d1(op, O, what, zeta, q, out, s);
+      case 2:
+        return post_d2(op, O, what, zeta, q, out, s);
+      case 3:
+        return post_d3(op, O, what, zeta, q, out, s);
+      default:
+        return post_d4(op, O, what, zeta, q, out, s);
+    }
+}
+
+// per-batch results -> host, reduced in batch order
+static int fetch_batches(oc_dg *op, hipStream_t s, std::vector<double> &h)
+{
+  h.resize(op->NB);
+  HIP_TRY(hipMemcpyAsync(h.data(), op->d_post, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return OC_OK;
+}
+
+int oc_dg_velocity(oc_dg *op, const double *zeta, const double *q, double *vel, void *stream)
+{
+  if (!op || !zeta || !q || !vel)
+    return bad_argument(__func__);
+  return post_dispatch(op, POST_VELOCITY, zeta, q, vel, (hipStream_t)stream);
+}
+
+int oc_dg_transport_speed(oc_dg *op, const double *zeta, const double *q, double *speed,
+                          void *stream)
+{
+  if (!op || !zeta || !q || !speed)
+    return bad_argument(__func__);
+  hipStream_t s = (hipStream_t)stream;
+  if (const int rc = post_dispatch(op, POST_SPEED, zeta, q, op->d_post, s))
+    return rc;
+  std::vector<double> h;
+  if (const int rc = fetch_batches(op, s, h))
+    return rc;
+  double m = 0.0;
+  for (double v : h)
+    m = std::max(m, v);
+  *speed = m;
+  return OC_OK;
+}
+
+int oc_dg_volume(oc_dg *op, const double *zeta, double *volume, void *stream)
+{
+  if (!op || !zeta || !volume)
+    return bad_argument(__func__);
+  hipStream_t s = (hipStream_t)stream;
+  if (const int rc = post_dispatch(op, POST_VOLUME, zeta, nullptr, op->d_post, s))
+    return rc;
+  std::vector<double> h;
+  if (const int rc = fetch_batches(op, s, h))
+    return rc;
+  double sum = 0.0;
+  for (double v : h)
+    sum += v;
+  *volume = sum;
+  return OC_OK;
+}
+
+int oc_dg_sum(oc_dg *op, const double *x, int32_t n_components, double *sum, void *stream)
+{
+  if (!op || !x || !sum || n_components < 1)
+    return bad_argument(__func__);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_batch_sum, dim3(op->NB), dim3(CB), 0, s, op->N, n_components * op->nd, x,
+                     op->d_post);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h;
+  if (const int rc = fetch_batches(op, s, h))
+    return rc;
+  double acc = 0.0;
+  for (double v : h)
+    acc += v;
+  *sum = acc;
   return OC_OK;
 }
 

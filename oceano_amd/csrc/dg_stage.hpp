@@ -996,8 +996,10 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
 // depth floor cannot bind and the capped Jacobi iteration is linear in the increment:
 // delta^K = sum_{k<K} (I - D^-1 M)^k D^-1 r with r = sum_j a_j L_zeta,j, a fixed matrix
 // (on the unit cell) times r / area.  Cells near the floor take the FP64 sweep.
-// Returns the sweep count reported for the cell (max_it on the closed form).
-template <int P>
+// Returns the sweep count reported for the cell (max_it on the closed form).  DEFER: a
+// cell that needs the FP64 sweep is left unchanged and flagged in O.defer for
+// k_depth_deferred (keeps the FP64 sweep's registers out of the fused FP32 stage kernel).
+template <int P, bool DEFER>
 __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTables &T,
                                                  const Phys<double> &M, int batch, int lane,
                                                  double area, const double *r, double *Z)
@@ -1054,13 +1056,18 @@ __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTa
       delta[i] = s * ia;
       dmax = fmaxf(dmax, fabsf(delta[i]));
     }
-  if (hmin > float(O.wet_margin) * dmax && hmin > 1e-6f)
+  const bool closed = hmin > float(O.wet_margin) * dmax && hmin > 1e-6f;
+  if (DEFER)
+    O.defer[batch * CB + lane] = closed ? 0 : 1;
+  if (closed)
     {
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         Z[i] += double(delta[i]);
       return O.max_it;
     }
+  if (DEFER)
+    return 0;
   // FP64 reference path: front = r + M h(zeta^n), then the capped sweeps
   double F[ND];
 #pragma unroll
@@ -1118,7 +1125,7 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
             F[i] += fj * kp[(size_t)i * CB];
         }
       if (centred<R>())
-        k = mixed_depth_solve<P>(O, T, M, batch, lane, area, F, Z);
+        k = mixed_depth_solve<P, false>(O, T, M, batch, lane, area, F, Z);
       else
         {
           unsigned long long bits = 0;
@@ -1135,6 +1142,79 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
         S.rk_dst[vix(batch, 1, 0, ND, i, lane)] = Z[i];
     }
   store_sweeps(S.sweeps_blk, batch, k);
+}
+
+// The FP64 depth solve of the cells the fused mixed-precision stage flagged (near the
+// depth floor): front from ki_h[1..stage+1] (all written by then) and zeta^n = w_h (the
+// stage leaves these cells' w_h untouched); adds their sweeps to the batch count.
+template <int P>
+__device__ __forceinline__ void depth_deferred_batch(const OpArgs &O, const StageArgs &S,
+                                                     int batch)
+{
+  constexpr int ND = (P + 1) * (P + 1);
+  const DegreeTables &T = basis<double>(P);
+  const int lane = threadIdx.x;
+  const bool mine = batch * CB + lane < O.n_cells && O.defer[batch * CB + lane] != 0;
+  if (!__any(mine))
+    return;
+  int k = 0;
+  if (mine)
+    {
+      const Phys<double> M = physics<double>(O);
+      const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
+      const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
+      const double area = hx * hy;
+      double F[ND], Z[ND];
+      const size_t base = (size_t)batch * ND * CB + lane;
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        {
+          F[i] = S.factor[0] * S.kprev[0][base + (size_t)i * CB];
+          Z[i] = S.wsol[base + (size_t)i * CB];
+        }
+      for (int j = 1; j <= S.stage; ++j)
+        {
+          const double *kp = S.kprev[j] + base;
+          const double fj = S.factor[j];
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            F[i] += fj * kp[(size_t)i * CB];
+        }
+      unsigned long long bits = 0;
+      double hdummy;
+      depth_mass<P, false>(O, T, M, batch, lane, Z, F, 1.0, area, bits, hdummy);
+      k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        S.rk_dst[base + (size_t)i * CB] = Z[i];
+    }
+  if (S.sweeps_blk)
+    {
+      unsigned int v = (unsigned int)k;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off);
+      if (lane == 0)
+        S.sweeps_blk[batch] += v;
+    }
+}
+
+// The FP64 depth solve of the cells the fused mixed-precision stage flagged (near the
+// depth floor): front from ki_h[1..stage+1] (all written by then) and zeta^n = w_h (the
+// stage leaves these cells' w_h untouched); adds their sweeps to the batch count.  One
+// wave looks after DEFER_BATCHES consecutive batches, so that a stage without flagged
+// cells (the common case) costs a short launch.
+constexpr int DEFER_BATCHES = 4;
+template <int P>
+__global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
+{
+  for (int i = 0; i < DEFER_BATCHES; ++i)
+    {
+      const int batch = blockIdx.x * DEFER_BATCHES + i;
+      if (batch >= O.n_batches)
+        break;
+      depth_deferred_batch<P>(O, S, batch);
+    }
 }
 
 // L_zeta, front and depth solve of one batch of cells in one pass: L_zeta stays in
@@ -1209,7 +1289,7 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
         Z[i] = S.wsol[vix(batch, 1, 0, ND, i, lane)];
       issue_barrier();
       if (centred<R>())
-        k = mixed_depth_solve<P>(O, T, M, batch, lane, area, F, Z);
+        k = mixed_depth_solve<P, true>(O, T, M, batch, lane, area, F, Z);
       else
         {
           unsigned long long bits = 0;

@@ -38,4 +38,10 @@ int rhs_apply_d2(oc_dg *op, const OpArgs &O, bool invert, const double *src_h,
   return launch_rhs_apply<2, double>(op, O, invert, src_h, src_q, src_v, dst_h, dst_q, s);
 }
 
+int post_d2(oc_dg *op, const OpArgs &O, int what, const double *zeta, const double *q,
+             double *out, hipStream_t s)
+{
+  return launch_post<2>(op, O, what, zeta, q, out, s);
+}
+
 }  // namespace oceano

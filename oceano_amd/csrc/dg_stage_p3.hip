@@ -38,4 +38,10 @@ int rhs_apply_d3(oc_dg *op, const OpArgs &O, bool invert, const double *src_h,
   return launch_rhs_apply<3, double>(op, O, invert, src_h, src_q, src_v, dst_h, dst_q, s);
 }
 
+int post_d3(oc_dg *op, const OpArgs &O, int what, const double *zeta, const double *q,
+             double *out, hipStream_t s)
+{
+  return launch_post<3>(op, O, what, zeta, q, out, s);
+}
+
 }  // namespace oceano

@@ -14,8 +14,12 @@
 
 #include "oceano_dg.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
+#include <array>
+#include <map>
 #include <random>
 #include <string>
 #include <unordered_map>
@@ -284,5 +288,237 @@ int oc_mesh_export(const oc_mesh *m, double *cell_geom, int32_t *cell_level, int
 }
 
 void oc_mesh_destroy(oc_mesh *m) { delete m; }
+
+// Gmsh 4.1 ASCII reader for the Cartesian coarse meshes of the test decks
+// (tests/shallowWaterVortex/shallowWaterVortex_20x10cells.msh,
+// tests/channelFlow/channelFlow_10x2cells.msh), standing in for GridIn::read_msh
+// (main.cc:621-637).  Quadrilaterals (type 3) must be axis-aligned and tile a uniform
+// nx x ny grid; boundary ids are the first physical tag of the curve entity that holds each
+// boundary line (type 1), as deal.II assigns them, and must be constant along each side.
+// The cells come out lexicographic (x fastest) rather than in file order: cell order
+// changes no result of the operator (every cell's arithmetic is independent of it).
+int oc_mesh_read_msh(const char *path, int32_t periodic_x, int32_t periodic_y, oc_mesh **mesh)
+{
+  using oceano::set_error;
+  if (!path || !mesh)
+    {
+      set_error("oc_mesh_read_msh: null argument");
+      return OC_ERR_ARGUMENT;
+    }
+  std::ifstream in(path);
+  if (!in)
+    {
+      set_error(std::string("oc_mesh_read_msh: cannot open ") + path);
+      return OC_ERR_ARGUMENT;
+    }
+  auto fail = [&](const std::string &msg) {
+    set_error(std::string("oc_mesh_read_msh: ") + path + ": " + msg);
+    return OC_ERR_MESH;
+  };
+  std::map<int64_t, int> curve_tag;  // curve entity -> first physical tag (0 if none)
+  std::map<int64_t, std::pair<double, double>> node;
+  std::vector<std::array<int64_t, 4>> quads;
+  std::vector<std::array<int64_t, 3>> lines;  // node, node, curve entity
+  std::string tok;
+  bool version_ok = false;
+  while (in >> tok)
+    {
+      if (tok == "$MeshFormat")
+        {
+          double version;
+          int file_type, dsize;
+          in >> version >> file_type >> dsize;
+          if (file_type != 0)
+            return fail("only ASCII files are read");
+          version_ok = version >= 4.1 - 1e-9 && version < 5.0;
+          if (!version_ok)
+            return fail("format version 4.1 expected");
+        }
+      else if (tok == "$Entities")
+        {
+          int64_t np, nc, ns, nv;
+          in >> np >> nc >> ns >> nv;
+          for (int64_t i = 0; i < np; ++i)
+            {
+              int64_t tag, nphys, t;
+              double x, y, z;
+              in >> tag >> x >> y >> z >> nphys;
+              for (int64_t k = 0; k < nphys; ++k)
+                in >> t;
+            }
+          for (int64_t i = 0; i < nc; ++i)
+            {
+              int64_t tag, nphys, nb, t;
+              double bb[6];
+              in >> tag;
+              for (double &v : bb)
+                in >> v;
+              in >> nphys;
+              int first = 0;
+              for (int64_t k = 0; k < nphys; ++k)
+                {
+                  in >> t;
+                  if (k == 0)
+                    first = (int)t;
+                }
+              curve_tag[tag] = first;
+              in >> nb;
+              for (int64_t k = 0; k < nb; ++k)
+                in >> t;
+            }
+          for (int64_t i = 0; i < ns + nv; ++i)
+            {
+              int64_t tag, nphys, nb, t;
+              double bb[6];
+              in >> tag;
+              for (double &v : bb)
+                in >> v;
+              in >> nphys;
+              for (int64_t k = 0; k < nphys; ++k)
+                in >> t;
+              in >> nb;
+              for (int64_t k = 0; k < nb; ++k)
+                in >> t;
+            }
+        }
+      else if (tok == "$Nodes")
+        {
+          int64_t nblocks, nnodes, mn, mx;
+          in >> nblocks >> nnodes >> mn >> mx;
+          for (int64_t b = 0; b < nblocks; ++b)
+            {
+              int64_t dim, ent, param, n;
+              in >> dim >> ent >> param >> n;
+              std::vector<int64_t> tags(n);
+              for (auto &t : tags)
+                in >> t;
+              for (int64_t i = 0; i < n; ++i)
+                {
+                  double x, y, z, u;
+                  in >> x >> y >> z;
+                  for (int64_t k = 0; param && k < dim; ++k)
+                    in >> u;
+                  node[tags[i]] = {x, y};
+                }
+            }
+        }
+      else if (tok == "$Elements")
+        {
+          int64_t nblocks, nel, mn, mx;
+          in >> nblocks >> nel >> mn >> mx;
+          for (int64_t b = 0; b < nblocks; ++b)
+            {
+              int64_t dim, ent, type, n;
+              in >> dim >> ent >> type >> n;
+              const int nn = type == 1 ? 2 : type == 3 ? 4 : type == 15 ? 1 : -1;
+              if (nn < 0)
+                return fail("element type " + std::to_string(type) +
+                            " (only points, lines and quadrilaterals)");
+              for (int64_t i = 0; i < n; ++i)
+                {
+                  int64_t tag, v[4];
+                  in >> tag;
+                  for (int k = 0; k < nn; ++k)
+                    in >> v[k];
+                  if (type == 3)
+                    quads.push_back({v[0], v[1], v[2], v[3]});
+                  else if (type == 1)
+                    lines.push_back({v[0], v[1], ent});
+                }
+            }
+        }
+      if (!in)
+        return fail("truncated or malformed section " + tok);
+    }
+  if (!version_ok || quads.empty())
+    return fail("no $MeshFormat 4.1 header or no quadrilaterals");
+  // the grid lines
+  std::vector<double> xs, ys;
+  for (const auto &q : quads)
+    for (int64_t t : q)
+      {
+        auto it = node.find(t);
+        if (it == node.end())
+          return fail("element refers to an unknown node");
+        xs.push_back(it->second.first);
+        ys.push_back(it->second.second);
+      }
+  // grid lines: the decks' node coordinates scatter around them by ~1e-12 (mesher
+  // round-off); coordinates within 1e-9 of the domain size are one line, and the nodes are
+  // snapped to the uniform lines (cells the reference sees deviate from these rectangles
+  // by that round-off)
+  auto lines_of = [](std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    const double tol = 1e-9 * (v.back() - v.front());
+    std::vector<double> out;
+    for (double x : v)
+      if (out.empty() || x - out.back() > tol)
+        out.push_back(x);
+    return out;
+  };
+  xs = lines_of(xs);
+  ys = lines_of(ys);
+  const int nx = (int)xs.size() - 1, ny = (int)ys.size() - 1;
+  if (nx < 1 || ny < 1 || (int64_t)quads.size() != (int64_t)nx * ny)
+    return fail("the quadrilaterals do not tile a tensor grid");
+  const double dx = (xs.back() - xs.front()) / nx, dy = (ys.back() - ys.front()) / ny;
+  for (int i = 0; i <= nx; ++i)
+    if (std::fabs(xs[i] - (xs.front() + i * dx)) > 1e-9 * dx)
+      return fail("grid lines in x are not uniform");
+  for (int j = 0; j <= ny; ++j)
+    if (std::fabs(ys[j] - (ys.front() + j * dy)) > 1e-9 * dy)
+      return fail("grid lines in y are not uniform");
+  auto snap = [](double v, double v0, double d, long &k) {
+    k = std::lround((v - v0) / d);
+    return std::fabs(v - (v0 + k * d)) <= 1e-9 * d;
+  };
+  std::vector<char> seen((size_t)nx * ny, 0);
+  for (const auto &q : quads)
+    {
+      long ix[4], iy[4];
+      for (int k = 0; k < 4; ++k)
+        if (!snap(node[q[k]].first, xs.front(), dx, ix[k]) ||
+            !snap(node[q[k]].second, ys.front(), dy, iy[k]))
+          return fail("a node is off the grid lines");
+      const long x0 = *std::min_element(ix, ix + 4), x1 = *std::max_element(ix, ix + 4);
+      const long y0 = *std::min_element(iy, iy + 4), y1 = *std::max_element(iy, iy + 4);
+      int corners = 0;
+      for (int k = 0; k < 4; ++k)
+        corners |= 1 << ((ix[k] == x1) + 2 * (iy[k] == y1));
+      if (x1 - x0 != 1 || y1 - y0 != 1 || corners != 15)
+        return fail("a quadrilateral is not an axis-aligned grid cell");
+      if (x0 < 0 || x0 >= nx || y0 < 0 || y0 >= ny || seen[(size_t)y0 * nx + x0]++)
+        return fail("the quadrilaterals do not tile a tensor grid");
+    }
+  // boundary ids per side: 0 x = x0, 1 x = x1, 2 y = y0, 3 y = y1
+  int32_t bid[4] = {0, 0, 0, 0};
+  bool have[4] = {false, false, false, false};
+  const double tx = 1e-6 * dx, ty = 1e-6 * dy;
+  for (const auto &l : lines)
+    {
+      if (!node.count(l[0]) || !node.count(l[1]))
+        return fail("line refers to an unknown node");
+      const double xa = node[l[0]].first, ya = node[l[0]].second;
+      const double xb = node[l[1]].first, yb = node[l[1]].second;
+      int side = -1;
+      if (std::fabs(xa - xs.front()) < tx && std::fabs(xb - xs.front()) < tx)
+        side = 0;
+      else if (std::fabs(xa - xs.back()) < tx && std::fabs(xb - xs.back()) < tx)
+        side = 1;
+      else if (std::fabs(ya - ys.front()) < ty && std::fabs(yb - ys.front()) < ty)
+        side = 2;
+      else if (std::fabs(ya - ys.back()) < ty && std::fabs(yb - ys.back()) < ty)
+        side = 3;
+      if (side < 0)
+        continue;  // not on the outer boundary
+      const int id = curve_tag.count(l[2]) ? curve_tag[l[2]] : 0;
+      if (have[side] && bid[side] != id)
+        return fail("boundary ids vary along a side of the rectangle");
+      bid[side] = id;
+      have[side] = true;
+    }
+  return oc_mesh_create_rectangle(xs.front(), xs.back(), ys.front(), ys.back(), nx, ny,
+                                  periodic_x, periodic_y, bid, mesh);
+}
 
 }  // extern "C"

@@ -100,3 +100,48 @@ def butcher_rows(scheme, dt):
     S = len(bi)
     rows = [np.array(ai[s]) * dt for s in range(S - 1)] + [np.array(bi) * dt]
     return rows, ci
+
+
+class DeviceIntegrator:
+    """Device-resident time loop of the library (oc_rk_*): the S stages of a scheme with
+    integrator-owned k / r registers, optionally followed by the velocity update, launched
+    eagerly (step) or captured once into a HIP graph and replayed (capture / replay).
+    ExplicitRungeKuttaIntegrator.h:179-298 driven as in main.cc:1546-1577."""
+
+    def __init__(self, op, scheme="stage_4_order_4"):
+        import ctypes as C
+        from . import _capi
+        ai, bi, _ = _tableau(scheme)
+        rows = [v for s in range(len(bi) - 1) for v in ai[s]] + list(bi)
+        arr = (C.c_double * len(rows))(*rows)
+        h = C.c_void_p()
+        _capi.check(_capi.lib().oc_rk_create(op._h, len(bi), arr, C.byref(h)), "oc_rk_create")
+        self._h, self.op, self.n_stages = h, op, len(bi)
+
+    def __del__(self):
+        from . import _capi
+        if getattr(self, "_h", None):
+            _capi.lib().oc_rk_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def _p(t):
+        return None if t is None else t.data_ptr()
+
+    def step(self, dt, w_h, w_q, velocity=None):
+        from . import _capi
+        from .operator import _stream
+        _capi.check(_capi.lib().oc_rk_step(self._h, dt, self._p(w_h), self._p(w_q),
+                                           self._p(velocity), _stream()), "oc_rk_step")
+
+    def capture(self, dt, w_h, w_q, velocity=None):
+        """Record one step on torch's current stream (must not be the null stream)."""
+        from . import _capi
+        from .operator import _stream
+        _capi.check(_capi.lib().oc_rk_capture(self._h, dt, self._p(w_h), self._p(w_q),
+                                              self._p(velocity), _stream()), "oc_rk_capture")
+
+    def replay(self, n=1):
+        from . import _capi
+        from .operator import _stream
+        _capi.check(_capi.lib().oc_rk_replay(self._h, n, _stream()), "oc_rk_replay")

@@ -35,6 +35,14 @@ class Mesh:
               "oc_mesh_create_rectangle")
         return cls(h)
 
+    @classmethod
+    def read_msh(cls, path, periodic=(False, False)):
+        """Gmsh 4.1 deck of a uniform Cartesian grid (GridIn::read_msh, main.cc:621-637)."""
+        h = C.c_void_p()
+        check(lib().oc_mesh_read_msh(str(path).encode(), int(periodic[0]), int(periodic[1]),
+                                     C.byref(h)), "oc_mesh_read_msh")
+        return cls(h)
+
     def refine_global(self, times=1):
         check(lib().oc_mesh_refine_global(self._h, times), "oc_mesh_refine_global")
         self._export()

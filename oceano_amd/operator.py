@@ -195,6 +195,33 @@ class OceanoOperator:
         return {"l2_depth": err[0], "l2_discharge": err[1], "l2_hu": err[2], "l2_hv": err[3],
                 "linf_depth": err[4], "linf_hu": err[5], "linf_hv": err[6]}
 
+    def velocity(self, zeta, q, vel=None):
+        """evaluate_velocity_field (OceanDG.h:2600-2662) into vel (operator order, 2 comps)."""
+        vel = self.initialize_vector(1) if vel is None else vel
+        check(lib().oc_dg_velocity(self._h, _vp(zeta), _vp(q), _vp(vel), _stream()),
+              "oc_dg_velocity")
+        return vel
+
+    def transport_speed(self, zeta, q):
+        """compute_cell_transport_speed (OceanDG.h:2506-2580); synchronises."""
+        v = C.c_double()
+        check(lib().oc_dg_transport_speed(self._h, _vp(zeta), _vp(q), C.byref(v), _stream()),
+              "oc_dg_transport_speed")
+        return v.value
+
+    def volume(self, zeta):
+        """check_mass cell integral (OceanDG.h:2138-2184): integral of the depth."""
+        v = C.c_double()
+        check(lib().oc_dg_volume(self._h, _vp(zeta), C.byref(v), _stream()), "oc_dg_volume")
+        return v.value
+
+    def sum(self, vec, n_components):
+        """sum of the active entries of an operator-order vector; synchronises."""
+        v = C.c_double()
+        check(lib().oc_dg_sum(self._h, _vp(vec), n_components, C.byref(v), _stream()),
+              "oc_dg_sum")
+        return v.value
+
     def count_nonfinite(self, vec):
         n = np.zeros(1, np.int64)
         check(lib().oc_dg_count_nonfinite(self._h, _vp(vec), vec.numel(),

@@ -1389,4 +1389,108 @@ int ocr_errors(void *h, const double *zeta, const double *q, const double *exact
   return 0;
 }
 
+// evaluate_velocity_field (OceanDG.h:2600-2662), FE_DGQ degree >= 1 branch: velocity at the
+// Gauss(p+1) points with z_b from cell_2, then transform_from_q_points_to_basis (the inverse
+// of the Gauss(p+1) collocation).  zeta [N][nd], q and vel [N][2][nd], cell-major.
+int ocr_velocity_field(void *h, const double *zeta, const double *q, double *vel)
+{
+  Oracle *o = static_cast<Oracle *>(h);
+  const int n1 = o->n1, nd = o->nd;
+  std::vector<double> z(nd), qx(nd), qy(nd), vq(2 * nd);
+  for (int c = 0; c < o->N; ++c)
+    {
+      o->eval_values(zeta + (size_t)c * nd, o->Sn, z.data());
+      o->eval_values(q + (size_t)c * 2 * nd, o->Sn, qx.data());
+      o->eval_values(q + (size_t)c * 2 * nd + nd, o->Sn, qy.data());
+      for (int k = 0; k < nd; ++k)
+        {
+          const double zb = o->cell2[((size_t)c * nd + k) * 2];
+          const V2 v = o->model.velocity(z[k], V2{qx[k], qy[k]}, zb);
+          vq[2 * k] = v.x;
+          vq[2 * k + 1] = v.y;
+        }
+      std::vector<double> T(nd);
+      for (int d = 0; d < 2; ++d)
+        {
+          double *U = vel + ((size_t)c * 2 + d) * nd;
+          for (int b = 0; b < n1; ++b)
+            for (int i = 0; i < n1; ++i)
+              {
+                double s = 0;
+                for (int a = 0; a < n1; ++a)
+                  s += o->Sninv(i, a) * vq[2 * (a + n1 * b) + d];
+                T[i + n1 * b] = s;
+              }
+          for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i)
+              {
+                double s = 0;
+                for (int b = 0; b < n1; ++b)
+                  s += o->Sninv(j, b) * T[i + n1 * b];
+                U[i + n1 * j] = s;
+              }
+        }
+    }
+  return 0;
+}
+
+// compute_cell_transport_speed (OceanDG.h:2506-2580) at the GL(p+2) points with z_b from
+// cell_1, for axis-aligned cells (inverse Jacobian diag(1/hx, 1/hy)); the five power
+// iterations of the largest eigenvalue of J^-T J^-1 are kept as written.
+double ocr_transport_speed(void *h, const double *zeta, const double *q)
+{
+  Oracle *o = static_cast<Oracle *>(h);
+  const int m = o->ng, np = m * m, nd = o->nd;
+  std::vector<double> z(np), qx(np), qy(np);
+  double max_transport = 0.0;
+  for (int c = 0; c < o->N; ++c)
+    {
+      const double ij[2] = {1.0 / o->geom[4 * c + 2], 1.0 / o->geom[4 * c + 3]};
+      o->eval_values(zeta + (size_t)c * nd, o->Sg, z.data());
+      o->eval_values(q + (size_t)c * 2 * nd, o->Sg, qx.data());
+      o->eval_values(q + (size_t)c * 2 * nd + nd, o->Sg, qy.data());
+      double local_max = 0.0;
+      for (int k = 0; k < np; ++k)
+        {
+          const double zb = o->cell1[(size_t)c * np + k];
+          const V2 v = o->model.velocity(z[k], V2{qx[k], qy[k]}, zb);
+          const double cs[2] = {ij[0] * v.x, ij[1] * v.y};
+          double conv = 0.0;
+          for (int d = 0; d < 2; ++d)
+            conv = std::max(conv, std::fabs(cs[d]));
+          const double sos = std::sqrt(o->model.square_wavespeed(z[k], zb));
+          double ev[2] = {1.0, 1.0};
+          for (int i = 0; i < 5; ++i)
+            {
+              for (int d = 0; d < 2; ++d)
+                ev[d] = ij[d] * (ij[d] * ev[d]);
+              double nrm = 0.0;
+              for (int d = 0; d < 2; ++d)
+                nrm = std::max(nrm, std::fabs(ev[d]));
+              for (int d = 0; d < 2; ++d)
+                ev[d] /= nrm;
+            }
+          const double je[2] = {ij[0] * ev[0], ij[1] * ev[1]};
+          const double lam = std::sqrt((je[0] * je[0] + je[1] * je[1]) /
+                                       (ev[0] * ev[0] + ev[1] * ev[1]));
+          local_max = std::max(local_max, lam * sos + conv);
+        }
+      max_transport = std::max(max_transport, local_max);
+    }
+  return max_transport;
+}
+
+// check_mass (OceanDG.h:2138-2184), cell part: sum over the DoFs of
+// local_apply_cell_mass_height(zeta) = the integral of the depth at GL(p+2)^2
+double ocr_volume(void *h, const double *zeta)
+{
+  Oracle *o = static_cast<Oracle *>(h);
+  std::vector<double> I((size_t)o->N * o->nd, 0.0);
+  o->cell_mass_height(zeta, I.data());
+  double s = 0.0;
+  for (double v : I)
+    s += v;
+  return s;
+}
+
 }  // extern "C"

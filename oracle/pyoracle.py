@@ -50,6 +50,9 @@ def lib():
             "ocr_quadrature_points": (C.c_int, [C.c_void_p, C.c_int, PD]),
             "ocr_project_collocation": (C.c_int, [C.c_void_p, C.c_int, PD, PD]),
             "ocr_errors": (C.c_int, [C.c_void_p, PD, PD, PD, PD]),
+            "ocr_velocity_field": (C.c_int, [C.c_void_p, PD, PD, PD]),
+            "ocr_transport_speed": (C.c_double, [C.c_void_p, PD, PD]),
+            "ocr_volume": (C.c_double, [C.c_void_p, PD]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -214,3 +217,17 @@ class Oracle:
                 ri_h, ri_q = r_h.copy(), r_q.copy()
             self.perform_stage_hydro(s, scheme_rows[s][:s + 1], ri_h, ri_q, v, ki_h, ki_q,
                                      w_h, w_q, r_h, r_q)
+
+    def velocity_field(self, zeta, q):
+        """evaluate_velocity_field (OceanDG.h:2600-2662)"""
+        vel = self.zeros(2)
+        lib().ocr_velocity_field(self._h, _d(zeta), _d(q), _d(vel))
+        return vel
+
+    def transport_speed(self, zeta, q):
+        """compute_cell_transport_speed (OceanDG.h:2506-2580)"""
+        return lib().ocr_transport_speed(self._h, _d(zeta), _d(q))
+
+    def volume(self, zeta):
+        """check_mass cell integral (OceanDG.h:2138-2184): integral of the depth"""
+        return lib().ocr_volume(self._h, _d(zeta))

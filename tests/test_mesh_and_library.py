@@ -132,3 +132,47 @@ def test_vortex_and_channel_functions():
     # channel bed: parabolic bump of height 0.2 at x = 10 plus bumps within [0.02, 0.05]
     d = problems.channel_data(np.array([[10.0, 0.5], [5.0, 0.5]]), [0.02, 0, 0, 0, 1, 1, 0, 1], 7)
     assert abs(d[0, 0] + 0.2) < 1e-15 and d[1, 0] == 0.0 and d[0, 1] == 0.02
+
+
+# ------------------------------------------------------ Gmsh decks of the reference ----
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_read_vortex_deck():
+    """tests/shallowWaterVortex/shallowWaterVortex_20x10cells.msh (a fixture of the
+    reference, copied as data): the 20 x 10 grid on [0,2]x[0,1] (node coordinates within
+    ~1e-12 of the grid lines), boundary id 1 at x = 0 (curve 7), 2 at x = 2 (curve 5) and 0
+    on the y sides (curves 4, 6) -- the ids the vortex case's boundary conditions use."""
+    m = Mesh.read_msh(os.path.join(GOLDEN, "shallowWaterVortex_20x10cells.msh"))
+    r = Mesh.rectangle(0, 2, 0, 1, 20, 10, boundary_ids=(1, 2, 0, 0))
+    assert m.n_cells == 200
+    assert np.array_equal(m.geom, r.geom)
+    assert np.array_equal(m.inner, r.inner) and np.array_equal(m.boundary, r.boundary)
+    ids = {(int(f), int(b)) for _, f, b in m.boundary}
+    assert ids == {(0, 1), (1, 2), (2, 0), (3, 0)}
+    # the C1 configuration: the deck refined three times, periodic
+    c1 = Mesh.read_msh(os.path.join(GOLDEN, "shallowWaterVortex_20x10cells.msh"),
+                       periodic=(True, True)).refine_global(3)
+    assert c1.n_cells == 12800 and c1.boundary.shape[0] == 0
+
+
+def test_read_channel_deck():
+    """tests/channelFlow/channelFlow_10x2cells.msh: 10 x 2 cells on [0,100]x[0,1]."""
+    m = Mesh.read_msh(os.path.join(GOLDEN, "channelFlow_10x2cells.msh"))
+    assert m.n_cells == 20
+    assert np.allclose(m.geom[:, 2], 10.0) and np.allclose(m.geom[:, 3], 0.5)
+    sides = {}
+    for _, f, b in m.boundary:
+        sides.setdefault(int(f), set()).add(int(b))
+    assert all(len(v) == 1 for v in sides.values()) and set(sides) == {0, 1, 2, 3}
+
+
+def test_read_msh_errors(tmp_path):
+    bad = tmp_path / "tri.msh"
+    bad.write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Nodes\n1 3 1 3\n2 1 0 3\n1\n2\n3\n"
+                   "0 0 0\n1 0 0\n0 1 0\n$EndNodes\n$Elements\n1 1 1 1\n2 1 2 1\n1 1 2 3\n"
+                   "$EndElements\n")
+    with pytest.raises(RuntimeError, match="element type 2"):
+        Mesh.read_msh(str(bad))
+    with pytest.raises(RuntimeError, match="cannot open"):
+        Mesh.read_msh(str(tmp_path / "missing.msh"))

@@ -274,3 +274,62 @@ def test_vortex_convergence():
     order_h = np.log2(errs[0][0] / errs[1][0])
     order_q = np.log2(errs[0][1] / errs[1][1])
     assert order_h > 2.3 and order_q > 2.3, (errs, order_h, order_q)
+
+
+# --------------------------------------------- time-loop diagnostics (N1 / N3) ---------
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_velocity_field_reproduces_polynomial_velocity(p):
+    """evaluate_velocity_field: for a wet constant depth H (h^4 >= cu^4, so the
+    desingularised inverse depth is 1/H to round-off) and q = H u with u a polynomial of
+    degree p per direction, the collocation back to the nodal basis gives u's nodal
+    values exactly."""
+    m = Mesh.rectangle(0, 2, 0, 1, 4, 2, periodic=(True, True))
+    o = po.Oracle.from_mesh(m, p)
+    o.initialize_data_at_quadrature(lambda xy: np.zeros((len(xy), 5)))
+    H = 1.7
+    ux = lambda xy: 0.3 + xy[:, 0] ** p - 0.2 * xy[:, 1] * xy[:, 0]
+    uy = lambda xy: -0.1 + 0.5 * xy[:, 1] ** p
+    xy = o.quadrature_points("cell_2")
+    h = o.project(np.full(len(xy), H), 1)
+    q = o.project(np.ascontiguousarray(np.stack([H * ux(xy), H * uy(xy)], 1)), 2)
+    vel = o.velocity_field(h, q)
+    ref = o.project(np.ascontiguousarray(np.stack([ux(xy), uy(xy)], 1)), 2)
+    assert np.abs(vel - ref).max() < 1e-13
+
+
+def test_transport_speed_closed_form():
+    """compute_cell_transport_speed on a uniform state: the largest cell gives
+    lambda_max (1/h) sqrt(g H) + max(|U|/hx, |V|/hy); for square cells lambda_max = 1/h,
+    for an hx < hy cell the five power iterations give sqrt((1 + r^2 hx^2/hy^2)/(1 + r^2))/hx
+    with r = (hx/hy)^10."""
+    g, H, U, V = 9.81, 2.0, 0.7, -0.4
+    for nx, ny in ((8, 4), (8, 8)):
+        m = Mesh.rectangle(0, 2, 0, 1, nx, ny, periodic=(True, True))
+        o = po.Oracle.from_mesh(m, 2, g=g)
+        o.initialize_data_at_quadrature(lambda xy: np.zeros((len(xy), 5)))
+        n = o.N * o.nd
+        h = o.project(np.full(n, H), 1)
+        q = o.project(np.ascontiguousarray(np.tile([H * U, H * V], (n, 1))), 2)
+        hx, hy = 2.0 / nx, 1.0 / ny
+        r = (min(hx, hy) / max(hx, hy)) ** 10
+        hs, hl = min(hx, hy), max(hx, hy)
+        lam = np.sqrt((1 + r * r * hs * hs / (hl * hl)) / (1 + r * r)) / hs
+        # inverse_depth = sqrt(2) H / sqrt(2 H^4) = 1/H up to rounding
+        expect = lam * np.sqrt(g * H) + max(abs(U) / hx, abs(V) / hy)
+        assert abs(o.transport_speed(h, q) - expect) < 1e-12 * expect
+
+
+def test_volume_is_integral_of_depth():
+    """check_mass cell integral: zeta + z_b with a polynomial bed integrates exactly by
+    GL(p+2) (degree <= 2p+1); depth floor applies where the bed is above zeta."""
+    p = 2
+    m = Mesh.rectangle(0, 2, 0, 1, 6, 3, periodic=(True, True))
+    o = po.Oracle.from_mesh(m, p)
+    zb = lambda xy: 0.5 + 0.1 * xy[:, 0] - 0.2 * xy[:, 1] ** 2
+    o.initialize_data_at_quadrature(
+        lambda xy: np.concatenate([zb(xy)[:, None], np.zeros((len(xy), 4))], 1))
+    xy = o.quadrature_points("cell_2")
+    h = o.project(1.0 + 0.3 * xy[:, 0], 1)
+    # integral over [0,2]x[0,1] of 1 + 0.3x + 0.5 + 0.1x - 0.2y^2
+    expect = 2.0 * 1.5 + 0.4 * 2.0 - 0.2 * 2.0 / 3.0
+    assert abs(o.volume(h) - expect) < 1e-13
