@@ -6,12 +6,15 @@ globally 4 times, then 30% of the cells (std::mt19937_64 seed 20261202) refined 
 97,280 cells with 85,944 hanging subfaces, 4.67 M scalar DoF (zeta, hu, hv), periodic.
 A "step" is one explicit RK stage (oc_dg_perform_stage_hydro) of classic RK4 over the
 whole mesh; inputs are resident in HBM when timing starts.  The metric counts
-DoF x stages / second.
+DoF x stages / second.  The timed region runs the device-resident time loop: one RK4 step
+captured into a HIP graph and replayed (plus eager stages for a remainder), exactly
+--steps stages between two synchronisations.
 
 Extra fields: `roofline` for the kernel that takes most of the stage time (algorithmic
-HBM bytes per launch / mean launch time from HIP events on the launch stream, against
-8 TB/s); `cpu_baseline` times the CPU restatement of the same stage (single core) on a
-bounded sample of the same workload.
+HBM bytes per launch / mean launch time from HIP events recorded on the launch stream
+around each kernel in a second pass over the same number of stages launched eagerly,
+against 8 TB/s; `eager_value` is that pass's rate); `cpu_baseline` times the CPU
+restatement of the same stage (single core) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -26,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = "vortex_p3_hanging_97k"
+# HBM bytes per launch of the stage kernel on this workload, measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE scaled by a 512 MiB copy (profiles/k_stage_vortex_p3_hanging_97k.md)
+MEASURED_TRAFFIC = {"fp64": 259.9e6, "mixed": 260.3e6}
 
 
 def build_problem():
@@ -105,7 +111,7 @@ def main():
 
     import torch
     from oceano_amd import problems
-    from oceano_amd.integrator import butcher_rows
+    from oceano_amd.integrator import DeviceIntegrator, butcher_rows
 
     if args.slot_chunk is not None:
         from oceano_amd import _capi
@@ -116,52 +122,79 @@ def main():
 
     def run(precision):
         op, data, w_h, w_q = problems.setup_operator(P, precision=precision)
+        rk = DeviceIntegrator(op, "stage_4_order_4")
         ki_h = [op.initialize_vector(0) for _ in range(S + 1)]
         ki_q = [op.initialize_vector(1) for _ in range(S + 1)]
         r_h = [op.initialize_vector(0) for _ in range(2)]
         r_q = [op.initialize_vector(1) for _ in range(2)]
 
-        def stage(k):
+        def stage(k):  # eager stage k of the running step
             s = k % S
             ri = [w_h, w_q] if s == 0 else [r_h[s % 2], r_q[s % 2]]
             op.perform_stage_hydro(s, 0.0, rows[s][:s + 1], ri + [None, None], ki_h, ki_q,
                                    w_h, w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
 
-        k0 = args.warmup + (-args.warmup) % S  # warm up whole steps: timing starts at stage 0
-        for k in range(k0):
-            stage(k)
+        n_done = 0  # completed time steps
+        for _ in range(max(1, -(-args.warmup // S))):
+            rk.step(args.dt, w_h, w_q)
+            n_done += 1
+        # headline: the device-resident loop -- one captured RK step replayed, plus any
+        # remaining stages eagerly, exactly args.steps stages between two synchronisations
+        side = torch.cuda.Stream()
         torch.cuda.synchronize()
-        op.set_timing(True)
+        with torch.cuda.stream(side):
+            rk.capture(args.dt, w_h, w_q)
+            rk.replay(1)
+        side.synchronize()
+        n_done += 1
+        full, rest = divmod(args.steps, S)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for k in range(args.steps):
-            stage(k0 + k)
+        with torch.cuda.stream(side):
+            rk.replay(full)
+            for k in range(rest):
+                stage(k)
+        side.synchronize()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        rep = op.timing_report()
-        op.set_timing(False)
-        # complete the last time step so the state is at a step boundary
-        k = k0 + args.steps
+        n_done += full
+        k = rest
         while k % S:
             stage(k)
             k += 1
+        n_done += 1 if rest else 0
+        # per-kernel durations: the same stages launched eagerly with HIP events around
+        # each kernel on the launch stream
+        op.set_timing(True)
         torch.cuda.synchronize()
-        n_steps = k // S
-        err = op.compute_errors(w_h, w_q, P.exact, n_steps * args.dt)
+        t1 = time.perf_counter()
+        for k in range(args.steps):
+            stage(k)
+        torch.cuda.synchronize()
+        eager = time.perf_counter() - t1
+        rep = op.timing_report()
+        op.set_timing(False)
+        k = args.steps
+        while k % S:
+            stage(k)
+            k += 1
+        n_done += k // S
+        torch.cuda.synchronize()
+        err = op.compute_errors(w_h, w_q, P.exact, n_done * args.dt)
         static = {}  # vortex: all problem-data fields uniform, none stored
         per_stage = [algorithmic_bytes(op, s, static) for s in range(S)]
         kern = {}
         for name, (ms, n) in rep.items():
             if n == 0:
                 continue
-            b = sum(per_stage[(k0 + j) % S][name] for j in range(args.steps)) / args.steps
+            b = sum(per_stage[j % S][name] for j in range(args.steps)) / args.steps
             mean_ms = ms / n
             kern[name] = {"mean_ms": mean_ms, "launches": n, "alg_bytes": b,
                           "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
-        return {"op": op, "elapsed": elapsed, "kernels": kern, "errors": err,
+        return {"op": op, "elapsed": elapsed, "eager": eager, "kernels": kern, "errors": err,
                 "state": (op.unpack(w_h, 1).cpu().numpy(), op.unpack(w_q, 2).cpu().numpy()),
                 "nonfinite": op.count_nonfinite(w_h) + op.count_nonfinite(w_q),
-                "sweeps_last_stage": op.last_sweeps(), "t_end": n_steps * args.dt}
+                "sweeps_last_stage": op.last_sweeps(), "t_end": n_done * args.dt}
 
     if args.copy_mib:
         n = args.copy_mib * (1 << 20) // 8
@@ -200,11 +233,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": None},
+                     "traffic": MEASURED_TRAFFIC.get("mixed" if "mixed" in runs else "fp64")
+                     if dom == "stage" else None},
     }
     for p, r in runs.items():
         line[p] = {"value": dofs * args.steps / r["elapsed"],
-                   "ms_per_step": r["elapsed"] / args.steps * 1e3, "kernels": r["kernels"],
+                   "ms_per_step": r["elapsed"] / args.steps * 1e3,
+                   "eager_value": dofs * args.steps / r["eager"], "kernels": r["kernels"],
                    "l2_error_vs_exact": {k: float(v) for k, v in r["errors"].items()},
                    "t_end": r["t_end"], "nonfinite": r["nonfinite"],
                    "jacobi_sweeps_last_stage": r["sweeps_last_stage"]}

@@ -266,7 +266,9 @@ int oc_dg_last_sweeps(oc_dg *op, int64_t *sweeps, void *stream);
 /* Per-kernel timing with HIP events recorded on the launch stream around each kernel of
  * oc_dg_perform_stage_hydro: 0 = the one-pass stage kernel; on the split path taken when
  * outputs alias inputs, 1 = L_zeta, 2 = L_q + inverse mass, 3 = discharge RK update,
- * 4 = front + depth-mass Jacobi.  oc_dg_timing_report synchronises, returns the summed milliseconds
+ * 4 = front + depth-mass Jacobi (also the mixed path's FP64 solve of cells near the depth
+ * floor, which follows kernel 0).  Enabling creates a pool of events up front.
+ * oc_dg_timing_report synchronises, returns the summed milliseconds
  * and launch counts per kernel since the last report, and clears them.  The reference
  * times the same two parts with TimerOutput sections "rk_stage hydro - integrals L_h" and
  * "rk_stage hydro - inv mass + vec upd" (OceanDG.h:1796, 1829). */

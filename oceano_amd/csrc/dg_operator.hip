@@ -1179,6 +1179,13 @@ int oc_dg_set_timing(oc_dg *op, int32_t enable)
   if (!op)
     return bad_argument(__func__);
   op->timing = enable != 0;
+  // events are created here, not between the launches they time
+  while (op->timing && op->event_pool.size() < 4096)
+    {
+      hipEvent_t e = nullptr;
+      HIP_TRY(hipEventCreate(&e));
+      op->event_pool.push_back(e);
+    }
   return OC_OK;
 }
 
