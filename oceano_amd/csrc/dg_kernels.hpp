@@ -37,6 +37,7 @@ struct Basis
   R ilumped2[MAXN1][MAXN1];  // 1 / (lumped[i] lumped[j]): inverse lumped mass / area
   R M1[MAXN1][MAXN1];        // sum_a Sg[a][i] Sg[a][j] wg[a]: 1-D GL(ng) mass (exact)
   R Sn[MAXN1][MAXN1];        // l_i at Gauss(n1) points (velocity projection)
+  R Psub[2][MAXN1][MAXN1];   // l_k((x_j + s)/2), x_j the nodes: restriction to half s
   R lam;                     // max_{a,i} Sg[a][i]^2: bound of |phi_ij| at the GL(ng)^2 points
 };
 using DegreeTables = Basis<double>;

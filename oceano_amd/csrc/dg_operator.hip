@@ -66,6 +66,10 @@ DegreeTables make_tables(int p)
   for (int a = 0; a < n1; ++a)
     for (int i = 0; i < n1; ++i)
       t.Sn[a][i] = S[a * n1 + i];
+  for (int s = 0; s < 2; ++s)
+    for (int j = 0; j < n1; ++j)
+      for (int k = 0; k < n1; ++k)
+        t.Psub[s][j][k] = lagrange_value(nodes.x, k, 0.5 * (nodes.x[j] + s));
   const std::vector<double> Si = small_inverse(S, n1);  // Si[i][a]
   for (int i = 0; i < n1; ++i)
     for (int a = 0; a < n1; ++a)

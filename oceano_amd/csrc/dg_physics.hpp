@@ -37,13 +37,8 @@ __device__ __forceinline__ double rsqrt_nr(double x)
   r = __builtin_fma(-(x * y), y, 1.0);
   return __builtin_fma(0.5 * y, r, y);
 }
-// FP32 (mixed path): the hardware reciprocal square root refined once
-__device__ __forceinline__ float rsqrt_nr(float x)
-{
-  const float y = __builtin_amdgcn_rsqf(x);
-  const float r = __builtin_fmaf(-(x * y), y, 1.0f);
-  return __builtin_fmaf(0.5f * y, r, y);
-}
+// FP32 (mixed path): the hardware reciprocal square root (within 1 ulp), unrefined
+__device__ __forceinline__ float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // a . n for the axis-aligned normals of the kernels: with one component of n zero the
 // other product is the whole dot product (a.x * 1 + a.y * 0 == a.x for finite a); the

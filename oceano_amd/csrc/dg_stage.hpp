@@ -216,33 +216,76 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                 break;
               phase_barrier();
               const int fi = fids[f][s];
-              R zn_l[N1], xn_l[N1], yn_l[N1];
+              // both traces as nodal values on this (sub)face: the coarse side restricts
+              // its own trace to half s, the fine side the neighbour's to half nsub, so
+              // that every face point below uses the one (wave-uniform) table Sg
+              R zo_s[N1], xo_s[N1], yo_s[N1], zn_s[N1], xn_s[N1], yn_s[N1];
 #pragma unroll
               for (int k = 0; k < N1; ++k)
                 {
-                  zn_l[k] = R(zg[s][k] - zc2[s]);
-                  xn_l[k] = R(xg[s][k]);
-                  yn_l[k] = R(yg[s][k]);
+                  zo_s[k] = zo_l[k];
+                  xo_s[k] = xo_l[k];
+                  yo_s[k] = yo_l[k];
+                  zn_s[k] = R(zg[s][k] - zc2[s]);
+                  xn_s[k] = R(xg[s][k]);
+                  yn_s[k] = R(yg[s][k]);
+                }
+              auto restrict3 = [&](int half, R *z3, R *x3, R *y3) {
+                R zt[N1], xt[N1], yt[N1];
+#pragma unroll
+                for (int j = 0; j < N1; ++j)
+                  {
+                    R sz = 0, sx = 0, sy = 0;
+#pragma unroll
+                    for (int k = 0; k < N1; ++k)
+                      {
+                        sz += T.Psub[half][j][k] * z3[k];
+                        sx += T.Psub[half][j][k] * x3[k];
+                        sy += T.Psub[half][j][k] * y3[k];
+                      }
+                    zt[j] = sz;
+                    xt[j] = sx;
+                    yt[j] = sy;
+                  }
+#pragma unroll
+                for (int j = 0; j < N1; ++j)
+                  {
+                    z3[j] = zt[j];
+                    x3[j] = xt[j];
+                    y3[j] = yt[j];
+                  }
+              };
+              if (kind == FK_COARSE)
+                restrict3(s, zo_s, xo_s, yo_s);
+              else if (kind == FK_FINE)
+                {
+                  if (nsub)
+                    restrict3(1, zn_s, xn_s, yn_s);
+                  else
+                    restrict3(0, zn_s, xn_s, yn_s);
                 }
               const R zc2r = R(zc2[s]);
               const R len = (kind == FK_COARSE) ? R(0.5) * flen : flen;
+              R acc[NC][N1];
+#pragma unroll
+              for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                for (int k = 0; k < N1; ++k)
+                  acc[cc][k] = 0;
 #pragma unroll 1
               for (int a = 0; a < NG; ++a)
                 {
-                  R mo[N1];
                   R zo = 0, xo = 0, yo = 0, zn = 0, xn = 0, yn = 0;
 #pragma unroll
                   for (int k = 0; k < N1; ++k)
                     {
-                      mo[k] = (kind == FK_COARSE) ? T.Sgs[s][a][k] : T.Sg[a][k];
-                      const R mn =
-                        (kind == FK_FINE) ? (nsub ? T.Sgs[1][a][k] : T.Sgs[0][a][k]) : T.Sg[a][k];
-                      zo += mo[k] * zo_l[k];
-                      xo += mo[k] * xo_l[k];
-                      yo += mo[k] * yo_l[k];
-                      zn += mn * zn_l[k];
-                      xn += mn * xn_l[k];
-                      yn += mn * yn_l[k];
+                      const R m = T.Sg[a][k];
+                      zo += m * zo_s[k];
+                      xo += m * xo_s[k];
+                      yo += m * yo_s[k];
+                      zn += m * zn_s[k];
+                      xn += m * xn_s[k];
+                      yn += m * yn_s[k];
                     }
                   zo += zcr;
                   zn += zc2r;
@@ -257,8 +300,32 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                       const R v = val[cc] * JxW;
 #pragma unroll
                       for (int k = 0; k < N1; ++k)
-                        Racc[cc][face_dof<N1>(f, k)] += mo[k] * v;
+                        acc[cc][k] += T.Sg[a][k] * v;
                     }
+                }
+              // test functions of the cell's face: the transposed restriction on the
+              // coarse side
+              if (kind == FK_COARSE)
+                {
+#pragma unroll
+                  for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                    for (int k = 0; k < N1; ++k)
+                      {
+                        R t = 0;
+#pragma unroll
+                        for (int j = 0; j < N1; ++j)
+                          t += T.Psub[s][j][k] * acc[cc][j];
+                        Racc[cc][face_dof<N1>(f, k)] += t;
+                      }
+                }
+              else
+                {
+#pragma unroll
+                  for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                    for (int k = 0; k < N1; ++k)
+                      Racc[cc][face_dof<N1>(f, k)] += acc[cc][k];
                 }
             }
         }
@@ -1149,12 +1216,11 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
 // stage leaves these cells' w_h untouched); adds their sweeps to the batch count.
 template <int P>
 __device__ __forceinline__ void depth_deferred_batch(const OpArgs &O, const StageArgs &S,
-                                                     int batch)
+                                                     int batch, bool mine)
 {
   constexpr int ND = (P + 1) * (P + 1);
   const DegreeTables &T = basis<double>(P);
   const int lane = threadIdx.x;
-  const bool mine = batch * CB + lane < O.n_cells && O.defer[batch * CB + lane] != 0;
   if (!__any(mine))
     return;
   int k = 0;
@@ -1208,12 +1274,21 @@ constexpr int DEFER_BATCHES = 4;
 template <int P>
 __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 {
+  // the flags of all the wave's batches in one group of loads
+  bool mine[DEFER_BATCHES];
+#pragma unroll
+  for (int i = 0; i < DEFER_BATCHES; ++i)
+    {
+      const int cell = (blockIdx.x * DEFER_BATCHES + i) * CB + threadIdx.x;
+      mine[i] = cell < O.n_cells && O.defer[cell] != 0;
+    }
+  issue_barrier();
   for (int i = 0; i < DEFER_BATCHES; ++i)
     {
       const int batch = blockIdx.x * DEFER_BATCHES + i;
       if (batch >= O.n_batches)
         break;
-      depth_deferred_batch<P>(O, S, batch);
+      depth_deferred_batch<P>(O, S, batch, mine[i]);
     }
 }
 
