@@ -145,3 +145,29 @@ class DeviceIntegrator:
         from . import _capi
         from .operator import _stream
         _capi.check(_capi.lib().oc_rk_replay(self._h, n, _stream()), "oc_rk_replay")
+
+    def run_to_steady(self, w_h, w_q, dt, tol=1e-10, max_steps=200000, chunk=500,
+                      stream=None):
+        """Replays the captured-step graph until one step changes the state by less than
+        tol in relative L2 (the C3 criterion ||w^{n+1} - w^n|| / ||w^n||, zeta and q
+        together), checked every `chunk` steps.  Returns (steps, last relative change)."""
+        import torch
+        side = stream or torch.cuda.Stream()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            self.capture(dt, w_h, w_q)
+        side.synchronize()
+        steps, change = 0, float("inf")
+        while steps < max_steps:
+            with torch.cuda.stream(side):
+                self.replay(chunk - 1)
+                ph, pq = w_h.clone(), w_q.clone()
+                self.replay(1)
+                d = torch.sqrt(((w_h - ph) ** 2).sum() + ((w_q - pq) ** 2).sum())
+                n = torch.sqrt((ph ** 2).sum() + (pq ** 2).sum())
+                change = float(d / n)
+            steps += chunk
+            if not change == change or change < tol:  # NaN or converged
+                break
+        side.synchronize()
+        return steps, change
