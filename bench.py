@@ -1,7 +1,7 @@
 """Throughput of the per-stage DG right-hand side on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): travelling vortex at DG
-degree 3 on the seeded non-conforming mesh -- [0,2]x[0,1], 20x10 coarse cells refined
+Default workload (the configuration the headline metric is quoted on): travelling vortex
+at DG degree 3 on the seeded non-conforming mesh -- [0,2]x[0,1], 20x10 coarse cells refined
 globally 4 times, then 30% of the cells (std::mt19937_64 seed 20261202) refined once more:
 97,280 cells with 85,944 hanging subfaces, 4.67 M scalar DoF (zeta, hu, hv), periodic.
 A "step" is one explicit RK stage (oc_dg_perform_stage_hydro) of classic RK4 over the
@@ -9,6 +9,12 @@ whole mesh; inputs are resident in HBM when timing starts.  The metric counts
 DoF x stages / second.  The timed region runs the device-resident time loop: one RK4 step
 captured into a HIP graph and replayed (plus eager stages for a remainder), exactly
 --steps stages between two synchronisations.
+
+Other workloads (--workload): vortex_p1_160x80 (12,800 cells, p = 1, cache-resident),
+vortex_p1_819k (the same refined to 819,200 cells, 9.8 M DoF), channel_p2_250x10 (2,500
+cells, p = 2, discharge inflow / height outflow / walls / Manning / bumpy bed) and
+ensemble_64 (64 vortex instances of 1k-50k cells and degrees 1-4, each degree's instances
+advanced by one operator over the union of their meshes; 25.6 M DoF).
 
 Extra fields: `roofline` for the kernel that takes most of the stage time (algorithmic
 HBM bytes per launch / mean launch time from HIP events recorded on the launch stream
@@ -28,15 +34,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = "vortex_p3_hanging_97k"
-# HBM bytes per launch of the stage kernel on this workload, measured with rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE scaled by a 512 MiB copy (profiles/k_stage_vortex_p3_hanging_97k.md)
-MEASURED_TRAFFIC = {"fp64": 259.9e6, "mixed": 260.3e6}
+DEFAULT = "vortex_p3_hanging_97k"
+# HBM bytes per launch of the stage kernel on the default workload, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 512 MiB copy
+# (profiles/k_stage_vortex_p3_hanging_97k.md)
+MEASURED_TRAFFIC = {DEFAULT: {"fp64": 259.9e6, "mixed": 260.3e6}}
 
 
-def build_problem():
+def workload(name):
+    """-> (list of Problem, dt or ("cfl", C) per problem, degree label, note)"""
     from oceano_amd import problems
-    return problems.vortex(degree=3, refine=4, refine_fraction=0.3, seed=20261202)
+    if name == "vortex_p3_hanging_97k":
+        return [problems.vortex(degree=3, refine=4, refine_fraction=0.3, seed=20261202)], \
+            [2e-5], 3, "HBM-resident"
+    if name == "vortex_p1_160x80":
+        return [problems.vortex(degree=1, refine=3)], [2.8e-4], 1, \
+            "cache-resident (1.2 MB per state)"
+    if name == "vortex_p1_819k":
+        return [problems.vortex(degree=1, refine=6)], [3.5e-5], 1, "HBM-resident"
+    if name == "channel_p2_250x10":  # the channel deck's CFL
+        return [problems.channel(degree=2, nx=250, ny=10)], [("cfl", 0.075)], 2, \
+            "cache-resident (0.5 MB per state)"
+    if name == "ensemble_64":  # CFL 0.15 / (p + 1): RK4 stability of DG(p) in 2D
+        groups, _ = problems.ensemble(n_instances=64, seed=11)
+        return [groups[p] for p in sorted(groups)], \
+            [("cfl", 0.15 / (p + 1)) for p in sorted(groups)], "1-4", \
+            "HBM-resident; one operator per degree over the union of its instances"
+    raise SystemExit("unknown workload " + name)
 
 
 def algorithmic_bytes(op, stage, static):
@@ -93,8 +117,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed RK stages")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--dt", type=float, default=2e-5,
-                    help="RK4 step (CFL ~0.03 on the finest cells; 6.5e-5 is unstable)")
+    ap.add_argument("--workload", default=DEFAULT,
+                    choices=[DEFAULT, "vortex_p1_160x80", "vortex_p1_819k",
+                             "channel_p2_250x10", "ensemble_64"])
+    ap.add_argument("--dt", type=float, default=None,
+                    help="RK4 step (default: the workload's; 2e-5 on the default mesh, "
+                         "where 6.5e-5 is unstable)")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -116,85 +144,119 @@ def main():
     if args.slot_chunk is not None:
         from oceano_amd import _capi
         _capi.check(_capi.lib().oc_set_option(b"slot_chunk", args.slot_chunk), "oc_set_option")
-    P = build_problem()
-    rows, _ = butcher_rows("stage_4_order_4", args.dt)
-    S = len(rows)
+    groups, dts, degree_label, note = workload(args.workload)
+    if args.dt is not None:
+        dts = [args.dt] * len(groups)
+    S = 4
+    multi = len(groups) > 1
+    n_steps_timed = args.steps if not multi else S * (-(-args.steps // S))
 
     def run(precision):
-        op, data, w_h, w_q = problems.setup_operator(P, precision=precision)
-        rk = DeviceIntegrator(op, "stage_4_order_4")
-        ki_h = [op.initialize_vector(0) for _ in range(S + 1)]
-        ki_q = [op.initialize_vector(1) for _ in range(S + 1)]
-        r_h = [op.initialize_vector(0) for _ in range(2)]
-        r_q = [op.initialize_vector(1) for _ in range(2)]
+        G = []
+        for P, dt in zip(groups, dts):
+            op, data, w_h, w_q = problems.setup_operator(P, precision=precision)
+            if isinstance(dt, tuple):  # main.cc:1503-1505: dt = CFL / transport speed
+                dt = dt[1] / float("%.3g" % op.transport_speed(w_h, w_q))
+            rows, _ = butcher_rows("stage_4_order_4", dt)
+            G.append({"P": P, "op": op, "w_h": w_h, "w_q": w_q, "dt": dt, "rows": rows,
+                      "rk": DeviceIntegrator(op, "stage_4_order_4"),
+                      "ki_h": [op.initialize_vector(0) for _ in range(S + 1)],
+                      "ki_q": [op.initialize_vector(1) for _ in range(S + 1)],
+                      "r_h": [op.initialize_vector(0) for _ in range(2)],
+                      "r_q": [op.initialize_vector(1) for _ in range(2)], "steps": 0})
 
-        def stage(k):  # eager stage k of the running step
+        def stage(g, k):  # eager stage k of the running step of operator g
             s = k % S
+            w_h, w_q, r_h, r_q = g["w_h"], g["w_q"], g["r_h"], g["r_q"]
             ri = [w_h, w_q] if s == 0 else [r_h[s % 2], r_q[s % 2]]
-            op.perform_stage_hydro(s, 0.0, rows[s][:s + 1], ri + [None, None], ki_h, ki_q,
-                                   w_h, w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
+            g["op"].perform_stage_hydro(s, 0.0, g["rows"][s][:s + 1], ri + [None, None],
+                                        g["ki_h"], g["ki_q"], w_h, w_q, r_h[(s + 1) % 2],
+                                        r_q[(s + 1) % 2])
 
-        n_done = 0  # completed time steps
-        for _ in range(max(1, -(-args.warmup // S))):
-            rk.step(args.dt, w_h, w_q)
-            n_done += 1
-        # headline: the device-resident loop -- one captured RK step replayed, plus any
-        # remaining stages eagerly, exactly args.steps stages between two synchronisations
+        def finish_step(g, k):
+            while k % S:
+                stage(g, k)
+                k += 1
+            g["steps"] += k // S
+
+        for g in G:
+            for _ in range(max(1, -(-args.warmup // S))):
+                g["rk"].step(g["dt"], g["w_h"], g["w_q"])
+                g["steps"] += 1
+        # headline: the device-resident loop -- one captured RK step per operator replayed,
+        # plus any remaining stages eagerly, exactly n_steps_timed stages between two
+        # synchronisations
         side = torch.cuda.Stream()
         torch.cuda.synchronize()
         with torch.cuda.stream(side):
-            rk.capture(args.dt, w_h, w_q)
-            rk.replay(1)
+            for g in G:
+                g["rk"].capture(g["dt"], g["w_h"], g["w_q"])
+                g["rk"].replay(1)
+                g["steps"] += 1
         side.synchronize()
-        n_done += 1
-        full, rest = divmod(args.steps, S)
+        full, rest = divmod(n_steps_timed, S)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         with torch.cuda.stream(side):
-            rk.replay(full)
+            for _ in range(full):
+                for g in G:
+                    g["rk"].replay(1)
             for k in range(rest):
-                stage(k)
+                stage(G[0], k)
         side.synchronize()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        n_done += full
-        k = rest
-        while k % S:
-            stage(k)
-            k += 1
-        n_done += 1 if rest else 0
+        for g in G:
+            g["steps"] += full
+        if rest:
+            finish_step(G[0], rest)
         # per-kernel durations: the same stages launched eagerly with HIP events around
         # each kernel on the launch stream
-        op.set_timing(True)
+        for g in G:
+            g["op"].set_timing(True)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        for k in range(args.steps):
-            stage(k)
+        for k in range(n_steps_timed):
+            for g in G:
+                stage(g, k)
         torch.cuda.synchronize()
         eager = time.perf_counter() - t1
-        rep = op.timing_report()
-        op.set_timing(False)
-        k = args.steps
-        while k % S:
-            stage(k)
-            k += 1
-        n_done += k // S
-        torch.cuda.synchronize()
-        err = op.compute_errors(w_h, w_q, P.exact, n_done * args.dt)
-        static = {}  # vortex: all problem-data fields uniform, none stored
-        per_stage = [algorithmic_bytes(op, s, static) for s in range(S)]
         kern = {}
-        for name, (ms, n) in rep.items():
-            if n == 0:
-                continue
-            b = sum(per_stage[j % S][name] for j in range(args.steps)) / args.steps
-            mean_ms = ms / n
-            kern[name] = {"mean_ms": mean_ms, "launches": n, "alg_bytes": b,
-                          "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
-        return {"op": op, "elapsed": elapsed, "eager": eager, "kernels": kern, "errors": err,
-                "state": (op.unpack(w_h, 1).cpu().numpy(), op.unpack(w_q, 2).cpu().numpy()),
-                "nonfinite": op.count_nonfinite(w_h) + op.count_nonfinite(w_q),
-                "sweeps_last_stage": op.last_sweeps(), "t_end": n_done * args.dt}
+        for g in G:
+            op = g["op"]
+            rep = op.timing_report()
+            op.set_timing(False)
+            finish_step(g, n_steps_timed)
+            per_stage = [algorithmic_bytes(op, s, {}) for s in range(S)]
+            for name, (ms, n) in rep.items():
+                if n == 0:
+                    continue
+                e = kern.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0.0})
+                e["ms"] += ms
+                e["launches"] += n
+                e["bytes"] += sum(per_stage[j % S][name] for j in range(n_steps_timed))
+        torch.cuda.synchronize()
+        kernels = {}
+        for name, e in kern.items():
+            # per stage of the whole workload (over all its operators)
+            mean_ms = e["ms"] / n_steps_timed
+            b = e["bytes"] / n_steps_timed
+            kernels[name] = {"mean_ms": mean_ms, "launches": e["launches"], "alg_bytes": b,
+                             "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
+        out = {"elapsed": elapsed, "eager": eager, "kernels": kernels, "groups": G,
+               "nonfinite": sum(g["op"].count_nonfinite(g["w_h"]) +
+                                g["op"].count_nonfinite(g["w_q"]) for g in G),
+               "sweeps_last_stage": sum(g["op"].last_sweeps() for g in G)}
+        if all(g["P"].exact is not None for g in G):
+            errs = [g["op"].compute_errors(g["w_h"], g["w_q"], g["P"].exact,
+                                           g["steps"] * g["dt"]) for g in G]
+            out["errors"] = {k: float(np.sqrt(sum(e[k] ** 2 for e in errs)))
+                             if k.startswith("l2") else float(max(e[k] for e in errs))
+                             for k in errs[0]}
+            out["t_end"] = [g["steps"] * g["dt"] for g in G]
+        out["state"] = [(g["op"].unpack(g["w_h"], 1).cpu().numpy(),
+                         g["op"].unpack(g["w_q"], 2).cpu().numpy()) for g in G]
+        return out
 
     if args.copy_mib:
         n = args.copy_mib * (1 << 20) // 8
@@ -208,54 +270,65 @@ def main():
         del src, dst
     precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {p: run(p) for p in precisions}
-    head = runs["mixed"] if "mixed" in runs else runs["fp64"]
-    dofs = P.dofs()
-    value = dofs * args.steps / head["elapsed"]
+    head_p = "mixed" if "mixed" in runs else "fp64"
+    head = runs[head_p]
+    dofs = sum(P.dofs() for P in groups)
+    value = dofs * n_steps_timed / head["elapsed"]
     dom = max(head["kernels"], key=lambda k: head["kernels"][k]["mean_ms"])
     achieved = head["kernels"][dom]["alg_GBs"]
+    traffic = MEASURED_TRAFFIC.get(args.workload, {}).get(head_p) if dom == "stage" else None
     line = {
         "metric": "DoF-RK-stages/s on 1 GPU, mixed vs FP64 at matched L2 error; % HBM peak",
         "value": value,
         "unit": "DoF-RK-stages/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": n_steps_timed,
         "warmup": args.warmup,
-        "ms_per_step": head["elapsed"] / args.steps * 1e3,
+        "ms_per_step": head["elapsed"] / n_steps_timed * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if "mixed" in runs else "f64",
         "data": "synthetic",
-        "config": {"workload": WORKLOAD, "cells": P.mesh.n_cells,
-                   "hanging_subfaces": P.mesh.n_hanging, "dofs": dofs, "degree": 3,
-                   "scheme": "classic RK4 (Butcher), stage = step", "dt": args.dt,
-                   "precision": "mixed" if "mixed" in runs else "fp64"},
+        "config": {"workload": args.workload, "cells": sum(P.mesh.n_cells for P in groups),
+                   "hanging_subfaces": sum(getattr(P.mesh, "n_hanging", 0) for P in groups),
+                   "dofs": dofs, "degree": degree_label, "operators": len(groups),
+                   "scheme": "classic RK4 (Butcher), stage = step",
+                   "dt": [g["dt"] for g in head["groups"]] if multi else head["groups"][0]["dt"],
+                   "precision": head_p, "memory": note},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": MEASURED_TRAFFIC.get("mixed" if "mixed" in runs else "fp64")
-                     if dom == "stage" else None},
+                     "traffic": traffic},
     }
     for p, r in runs.items():
-        line[p] = {"value": dofs * args.steps / r["elapsed"],
-                   "ms_per_step": r["elapsed"] / args.steps * 1e3,
-                   "eager_value": dofs * args.steps / r["eager"], "kernels": r["kernels"],
-                   "l2_error_vs_exact": {k: float(v) for k, v in r["errors"].items()},
-                   "t_end": r["t_end"], "nonfinite": r["nonfinite"],
+        line[p] = {"value": dofs * n_steps_timed / r["elapsed"],
+                   "ms_per_step": r["elapsed"] / n_steps_timed * 1e3,
+                   "eager_value": dofs * n_steps_timed / r["eager"], "kernels": r["kernels"],
+                   "nonfinite": r["nonfinite"],
                    "jacobi_sweeps_last_stage": r["sweeps_last_stage"]}
+        if "errors" in r:
+            line[p]["l2_error_vs_exact"] = r["errors"]
+            line[p]["t_end"] = r["t_end"] if multi else r["t_end"][0]
     if len(runs) == 2:
-        (fh, fq), (mh, mq) = runs["fp64"]["state"], runs["mixed"]["state"]
-        nd = runs["fp64"]["op"].dofs_per_cell
-
         def rel(a, b):
             return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        fs, ms = runs["fp64"]["state"], runs["mixed"]["state"]
+        nds = [g["op"].dofs_per_cell for g in runs["fp64"]["groups"]]
+
+        def comp(states, c):
+            return np.concatenate([s[1].reshape(-1, 2, nd)[:, c].ravel()
+                                   for s, nd in zip(states, nds)])
+        qn = float(np.sqrt(np.linalg.norm(comp(fs, 0)) ** 2 + np.linalg.norm(comp(fs, 1)) ** 2))
         line["mixed_vs_fp64"] = {
             "speedup": line["mixed"]["value"] / line["fp64"]["value"],
-            "rel_l2_zeta": rel(mh, fh),
-            "rel_l2_hu": rel(mq.reshape(-1, 2, nd)[:, 0], fq.reshape(-1, 2, nd)[:, 0]),
-            "rel_l2_hv": rel(mq.reshape(-1, 2, nd)[:, 1], fq.reshape(-1, 2, nd)[:, 1])}
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(P, args.cpu_stages)
+            "rel_l2_zeta": rel(np.concatenate([s[0] for s in ms]),
+                               np.concatenate([s[0] for s in fs])),
+            # per component, relative to the norm of the whole discharge (hv ~ 0 in a channel)
+            "rel_l2_hu": float(np.linalg.norm(comp(ms, 0) - comp(fs, 0)) / qn),
+            "rel_l2_hv": float(np.linalg.norm(comp(ms, 1) - comp(fs, 1)) / qn)}
+    if not args.no_cpu and not multi:
+        line["cpu_baseline"] = cpu_baseline(groups[0], args.cpu_stages)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -141,3 +141,75 @@ def setup_operator(problem, precision="fp64", device="cuda", max_iteration_heigh
     zb2 = data["cell_2"][:, 0]
     h, q = op.project_hydro(problem.initial, zb2)
     return op, data, h, q
+
+
+class MeshUnion:
+    """Disjoint union of meshes as one operator mesh: the cell and face indices of instance
+    i are offset by the cells before it, so that one operator (one launch per stage)
+    advances every instance; no face joins two instances.  The instances keep their own
+    coordinates (the operator reads only cell sizes and connectivity); functions of the
+    union tell instances apart by the cell-major order of the points they receive."""
+
+    def __init__(self, meshes, x_shift=0.0):
+        geoms, inners, bdrys, starts = [], [], [], [0]
+        for i, m in enumerate(meshes):
+            g = m.geom.copy()
+            g[:, 0] += x_shift * i
+            geoms.append(g)
+            inn = m.inner.copy()
+            inn[:, 0] += starts[-1]
+            inn[:, 3] += starts[-1]
+            inners.append(inn)
+            b = m.boundary.copy()
+            b[:, 0] += starts[-1]
+            bdrys.append(b)
+            starts.append(starts[-1] + m.n_cells)
+        self.geom = np.ascontiguousarray(np.concatenate(geoms), np.float64)
+        self.inner = np.ascontiguousarray(np.concatenate(inners), np.int32)
+        self.boundary = np.ascontiguousarray(np.concatenate(bdrys), np.int32).reshape(-1, 3)
+        self.n_cells = starts[-1]
+        self.n_hanging = sum(getattr(m, "n_hanging", 0) for m in meshes)
+        self.starts = starts
+        self.x_shift = x_shift
+        self._keep = meshes
+
+    def ranges(self, n_points):
+        """row ranges of each instance in a cell-major array of n_points points"""
+        per = n_points // self.n_cells
+        return [(per * a, per * b) for a, b in zip(self.starts[:-1], self.starts[1:])]
+
+
+def ensemble(n_instances=64, seed=11, n_min=1000, n_max=50000, degrees=(1, 2, 3, 4),
+             g=9.81):
+    """C4: n_instances travelling vortices.  Per instance: N ~ log-uniform[n_min, n_max]
+    realised as the nearest 2:1 grid (2m x m cells) on [0,2]x[0,1], periodic; p ~ U(degrees);
+    centre ~ U([0.5,1.5]x[0.4,0.6]), radius ~ U[0.2,0.35], amplitude ~ U[0.05,0.15]
+    (numpy default_rng(seed)).  Returns {degree: Problem over the union of that degree's
+    instances} and the per-instance parameters."""
+    rng = np.random.default_rng(seed)
+    inst = []
+    for i in range(n_instances):
+        n = np.exp(rng.uniform(np.log(n_min), np.log(n_max)))
+        m = max(1, int(round(np.sqrt(n / 2.0))))
+        inst.append({"ny": m, "nx": 2 * m, "degree": int(rng.choice(degrees)),
+                     "center": (rng.uniform(0.5, 1.5), rng.uniform(0.4, 0.6)),
+                     "radius": rng.uniform(0.2, 0.35), "amplitude": rng.uniform(0.05, 0.15)})
+    groups = {}
+    for p in sorted({d["degree"] for d in inst}):
+        members = [d for d in inst if d["degree"] == p]
+        meshes = [Mesh.rectangle(0.0, 2.0, 0.0, 1.0, d["nx"], d["ny"], periodic=(True, True))
+                  for d in members]
+        U = MeshUnion(meshes)
+        prms = [[1.0, 1.0, 0.0, d["amplitude"], d["radius"], d["center"][0], d["center"][1], g,
+                 2.0, 1.0] for d in members]
+
+        def exact(xy, t, U=U, prms=prms):
+            out = np.zeros((xy.shape[0], 3))
+            for (a, b), prm in zip(U.ranges(xy.shape[0]), prms):
+                out[a:b] = vortex_exact(xy[a:b], t, prm)
+            return out
+
+        groups[p] = Problem(U, p, {}, lambda xy: np.zeros((xy.shape[0], 5)),
+                            lambda xy, exact=exact: exact(xy, 0.0), None, exact, g=g,
+                            name="ensemble_p%d" % p)
+    return groups, inst
