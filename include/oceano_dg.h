@@ -183,6 +183,30 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
                               uint32_t n_ki, double *w_h, double *w_q, double *r_h,
                               double *r_q, int32_t write_scratch, void *stream);
 
+/* One additive (IMEX) Runge-Kutta stage: the stiff Manning friction is implicit, the rest
+ * of the operator explicit.  Replaces the TIMEINTEGRATOR_ADDITIVERUNGEKUTTA overload of
+ * OceanoOperator::perform_stage_hydro (OceanDG.h:196-208, body 1936-2119) as called by
+ * AdditiveRungeKuttaIntegrator::perform_time_step (AdditiveRungeKuttaIntegrator.h:239-316).
+ *   factor[0..stage]        dt-scaled explicit row a_i (b_i at the last stage)
+ *   factor_tilde[0..stage]  dt-scaled implicit row a~_i (ignored at the last stage)
+ *   factor_matrix           d~_stage dt: the implicit diagonal (pde_operator.factor_matrix)
+ *   ki_h[0..n_ki_h-1]       n_ki_h = n_stages + 1: [stage+1] <- L_zeta
+ *   ki_q[0..n_ki_q-1]       n_ki_q = 2 n_stages: [2 stage+1] <- explicit L_q (without the
+ *                           friction source before the last stage), [2 stage+2] <- the
+ *                           friction source tested against the basis (not at the last stage)
+ *   r_h, r_q                next_ri (written before the last stage; must differ from ri_h,
+ *                           ri_q: the stage reads the input after writing r_h)
+ * The last stage updates w_h, w_q in place and writes r_q = M^-1 (b . kex + b . kim).
+ * Boundary data come from the stores set with oc_dg_set_boundary_values, so `time` is not
+ * read.  ri_vel is read only when the viscosity is nonzero. */
+int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
+                                  const double *factor, const double *factor_tilde,
+                                  uint32_t n_factor, double factor_matrix, const double *ri_h,
+                                  const double *ri_q, const double *ri_vel,
+                                  double *const *ki_h, uint32_t n_ki_h, double *const *ki_q,
+                                  uint32_t n_ki_q, double *w_h, double *w_q, double *r_h,
+                                  double *r_q, void *stream);
+
 /* dst = M^-1 L(src): step-67's OceanoOperator::apply (declared OceanDG.h:178-180; its
  * body in the reference is dead code).  Both fields use the exact tensor-product inverse
  * of the mass matrix (affine cells); L is the spatial operator of perform_stage_hydro. */

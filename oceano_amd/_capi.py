@@ -98,6 +98,9 @@ SIGNATURES = {
     "oc_dg_unpack": (C.c_int, [P, I32, P, P, P]),
     "oc_dg_perform_stage_hydro": (C.c_int, [P, U32, D, PD, U32, P, P, P, C.POINTER(P),
                                             C.POINTER(P), U32, P, P, P, P, I32, P]),
+    "oc_dg_perform_stage_hydro_ark": (C.c_int, [P, U32, D, PD, PD, U32, D, P, P, P,
+                                                C.POINTER(P), U32, C.POINTER(P), U32, P, P,
+                                                P, P, P]),
     "oc_dg_apply": (C.c_int, [P, D, P, P, P, P, P, P]),
     "oc_dg_rhs": (C.c_int, [P, P, P, P, P, P, P]),
     "oc_dg_count_nonfinite": (C.c_int, [P, P, I64, PI64, P]),

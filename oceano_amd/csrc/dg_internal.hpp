@@ -42,6 +42,8 @@ struct oc_dg
   double c_cell1 = 0;
   double *d_cell2 = nullptr;
   double c_cell2 = 0;
+  double *d_cell2n = nullptr;
+  double c_cell2n = 0;
   double *d_face = nullptr;
   double c_face = 0;
   double *d_bz = nullptr;
@@ -74,6 +76,7 @@ struct oc_dg
       o.cell0[f] = {d_cell0[f], c_cell0[f]};
     o.cell1 = {d_cell1, c_cell1};
     o.cell2 = {d_cell2, c_cell2};
+    o.cell2n = {d_cell2n, c_cell2n};
     o.facez = {d_face, c_face};
     o.bdryz = {d_bz, c_bz};
     o.bvals = d_bvals;
@@ -120,7 +123,12 @@ namespace oceano {
                      const double *src_q, const double *src_v, double *dst_h,              \
                      double *dst_q, hipStream_t s);                                        \
   int post_d##P(oc_dg *op, const OpArgs &O, int what, const double *zeta, const double *q,  \
-                double *out, hipStream_t s);
+                double *out, hipStream_t s);                                               \
+  int stage_ark_d##P(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stages,         \
+                     const double *factor, const double *factor_tilde, double fm,          \
+                     const double *ri_h, const double *ri_q, const double *ri_v,           \
+                     double *const *ki_h, double *const *ki_q, double *w_h, double *w_q,   \
+                     double *r_h, double *r_q, hipStream_t s);
 // per-degree post-processing kernels (post_dP)
 enum PostKernel
 {

@@ -67,6 +67,7 @@ struct OpArgs
   Field cell0[5];       // [batch][nq^2][lane] z_b, manning, tau_x, tau_y, coriolis
   Field cell1;          // [batch][ng^2][lane] z_b at GL points
   Field cell2;          // [batch][n1^2][lane] z_b at Gauss(n1) points
+  Field cell2n;         // [batch][n1^2][lane] friction coefficient at Gauss(n1) points
   Field facez;          // [face][ng][2]
   Field bdryz;          // [bface][nq]
   const double *bvals;  // [bface][nq][3]

@@ -1,6 +1,7 @@
 // dg_launch.hpp -- launch sequences of one stage / rhs / apply, per degree.
 #pragma once
 
+#include "dg_ark.hpp"
 #include "dg_internal.hpp"
 #include "dg_post.hpp"
 #include "dg_stage.hpp"

@@ -337,6 +337,7 @@ void free_all(oc_dg *op)
     f(p);
   f(op->d_cell1);
   f(op->d_cell2);
+  f(op->d_cell2n);
   f(op->d_face);
   f(op->d_bz);
   f(op->d_bvals);
@@ -879,10 +880,15 @@ int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
       case OC_STORE_CELL_COLLOC:
         {
           op->cell2_host.assign(values, values + (size_t)N * op->nd * 2);
-          std::vector<double> cm((size_t)N * op->nd);
+          std::vector<double> cm((size_t)N * op->nd), cn((size_t)N * op->nd);
           for (size_t i = 0; i < cm.size(); ++i)
-            cm[i] = values[2 * i];
-          return interleave(cm, op->nd, &op->d_cell2, op->c_cell2);
+            {
+              cm[i] = values[2 * i];
+              cn[i] = values[2 * i + 1];
+            }
+          if ((rc = interleave(cm, op->nd, &op->d_cell2, op->c_cell2)))
+            return rc;
+          return interleave(cn, op->nd, &op->d_cell2n, op->c_cell2n);
         }
       case OC_STORE_FACE:
         {
@@ -1039,6 +1045,60 @@ static int rhs_or_apply(oc_dg *op, bool invert, const double *src_h, const doubl
         return rhs_apply_d3(op, O, invert, src_h, src_q, src_vel, dst_h, dst_q, s);
       default:
         return rhs_apply_d4(op, O, invert, src_h, src_q, src_vel, dst_h, dst_q, s);
+    }
+}
+
+int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
+                                  const double *factor, const double *factor_tilde,
+                                  uint32_t n_factor, double factor_matrix, const double *ri_h,
+                                  const double *ri_q, const double *ri_vel,
+                                  double *const *ki_h, uint32_t n_ki_h, double *const *ki_q,
+                                  uint32_t n_ki_q, double *w_h, double *w_q, double *r_h,
+                                  double *r_q, void *stream)
+{
+  (void)time;  // boundary data are tabulated by the caller (oc_dg_set_boundary_values)
+  if (!op || !factor || !ri_h || !ri_q || !ki_h || !ki_q || !w_h || !w_q || !r_q)
+    {
+      set_error("oc_dg_perform_stage_hydro_ark: null argument");
+      return OC_ERR_ARGUMENT;
+    }
+  const uint32_t S = n_ki_h - 1;
+  if (n_ki_h < 2 || n_ki_q != 2 * S || stage >= S || n_factor != stage + 1 || S + 1 > MAXKI)
+    {
+      set_error("oc_dg_perform_stage_hydro_ark: stage / n_ki_h / n_ki_q / n_factor "
+                "inconsistent (n_ki_q = 2 (n_ki_h - 1))");
+      return OC_ERR_ARGUMENT;
+    }
+  const bool last = stage == S - 1;
+  if ((!last && (!r_h || !factor_tilde || r_h == ri_h || r_q == ri_q)) ||
+      (op->nu != 0.0 && !ri_vel))
+    {
+      set_error("oc_dg_perform_stage_hydro_ark: r_h and factor_tilde are required before the "
+                "last stage, r distinct from ri, the velocity when viscosity != 0");
+      return OC_ERR_ARGUMENT;
+    }
+  for (uint32_t j = 1; j < n_ki_q; ++j)  // [0] (the front) is not written here
+    if (!ki_q[j] || (j <= S && !ki_h[j]))
+      {
+        set_error("oc_dg_perform_stage_hydro_ark: null ki vector");
+        return OC_ERR_ARGUMENT;
+      }
+  const OpArgs O = op->args();
+  hipStream_t s = (hipStream_t)stream;
+  switch (op->P)
+    {
+      case 1:
+        return stage_ark_d1(op, O, stage, S, factor, factor_tilde, factor_matrix, ri_h, ri_q,
+                            ri_vel, ki_h, ki_q, w_h, w_q, r_h, r_q, s);
+      case 2:
+        return stage_ark_d2(op, O, stage, S, factor, factor_tilde, factor_matrix, ri_h, ri_q,
+                            ri_vel, ki_h, ki_q, w_h, w_q, r_h, r_q, s);
+      case 3:
+        return stage_ark_d3(op, O, stage, S, factor, factor_tilde, factor_matrix, ri_h, ri_q,
+                            ri_vel, ki_h, ki_q, w_h, w_q, r_h, r_q, s);
+      default:
+        return stage_ark_d4(op, O, stage, S, factor, factor_tilde, factor_matrix, ri_h, ri_q,
+                            ri_vel, ki_h, ki_q, w_h, w_q, r_h, r_q, s);
     }
 }
 

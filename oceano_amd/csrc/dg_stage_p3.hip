@@ -44,4 +44,17 @@ int post_d3(oc_dg *op, const OpArgs &O, int what, const double *zeta, const doub
   return launch_post<3>(op, O, what, zeta, q, out, s);
 }
 
+int stage_ark_d3(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stages,
+                  const double *factor, const double *factor_tilde, double fm,
+                  const double *ri_h, const double *ri_q, const double *ri_v,
+                  double *const *ki_h, double *const *ki_q, double *w_h, double *w_q,
+                  double *r_h, double *r_q, hipStream_t s)
+{
+  if (op->precision == OC_PRECISION_MIXED)
+    return launch_stage_ark<3, float>(op, O, stage, n_stages, factor, factor_tilde, fm, ri_h,
+                                       ri_q, ri_v, ki_h, ki_q, w_h, w_q, r_h, r_q, s);
+  return launch_stage_ark<3, double>(op, O, stage, n_stages, factor, factor_tilde, fm, ri_h,
+                                      ri_q, ri_v, ki_h, ki_q, w_h, w_q, r_h, r_q, s);
+}
+
 }  // namespace oceano

@@ -94,6 +94,67 @@ class ExplicitRungeKuttaIntegrator:
                                    self.r_q[nxt])
 
 
+def ark_tableau(name="stage_3_order_2"):
+    """Additive (IMEX) RK of AdditiveRungeKuttaIntegrator.h:149-175: explicit
+    THREE_STAGE_SECOND_ORDER (time_stepping.h:223-245) with the implicit TRAPEZOIDAL_BDF2
+    (time_stepping.h:340-379, Giraldo et al. 2012).  Returns (a, b, c, a_tilde, d_tilde):
+    a, a_tilde the strictly lower rows feeding stages 1.., d_tilde the diagonal implicit
+    coefficients of stages 0..S-2 (the last stage has no implicit solve)."""
+    if name != "stage_3_order_2":
+        raise ValueError("unknown additive RK scheme " + name)
+    chi, a32 = 2.0 - np.sqrt(2.0), 0.5
+    a = [[chi], [1.0 - a32, a32]]
+    b = [0.5 - chi / 4.0, 0.5 - chi / 4.0, chi / 2.0]
+    c = [0.0, chi, 1.0]
+    at = [[chi / 2.0], [1.0 / (2.0 * np.sqrt(2.0)), 1.0 / (2.0 * np.sqrt(2.0))]]
+    dt_ = [chi / 2.0, 1.0 - 1.0 / np.sqrt(2.0)]
+    return a, b, c, at, dt_
+
+
+class AdditiveRungeKuttaIntegrator:
+    """IMEX time step of AdditiveRungeKuttaIntegrator::perform_time_step
+    (AdditiveRungeKuttaIntegrator.h:207-339): the stiff Manning friction implicit, the rest
+    explicit, one perform_stage_hydro_ark call per stage with factor = a[stage]
+    (b at the last stage), factor_tilde = a~[stage] (b at the last stage) and the implicit
+    diagonal factor_matrix = d~[stage] dt.  The reference reads d~[S-1] at the last stage,
+    one past the two coefficients TRAPEZOIDAL_BDF2 defines; the last stage has no implicit
+    solve, so 0 is passed there.  Two r registers alternate as in the explicit integrator."""
+
+    def __init__(self, scheme="stage_3_order_2"):
+        self.ai, self.bi, self.ci, self.ati, self.dti = ark_tableau(scheme)
+        self.scheme = scheme
+
+    def n_stages(self):
+        return len(self.bi)
+
+    def reinit(self, op):
+        """r registers and vec_ki: n_stages+1 height and 2 n_stages discharge vectors
+        (AdditiveRungeKuttaIntegrator.h:158-182)."""
+        S = self.n_stages()
+        self.r_h = [op.initialize_vector(0), op.initialize_vector(0)]
+        self.r_q = [op.initialize_vector(1), op.initialize_vector(1)]
+        self.ki_h = [op.initialize_vector(0) for _ in range(S + 1)]
+        self.ki_q = [op.initialize_vector(1) for _ in range(2 * S)]
+
+    def perform_time_step(self, op, current_time, time_step, solution_height,
+                          solution_discharge, postprocess_velocity=None):
+        S = self.n_stages()
+        b = [v * time_step for v in self.bi]
+        for stage in range(S):
+            last = stage == S - 1
+            fac = b if last else [v * time_step for v in self.ai[stage]]
+            fat = b if last else [v * time_step for v in self.ati[stage]]
+            fm = self.dti[stage] * time_step if stage < len(self.dti) else 0.0
+            if stage == 0:
+                ri = [solution_height, solution_discharge, None, postprocess_velocity]
+            else:
+                ri = [self.r_h[stage % 2], self.r_q[stage % 2], None, postprocess_velocity]
+            nxt = (stage + 1) % 2
+            op.perform_stage_hydro_ark(stage, current_time + self.ci[stage] * time_step, fac,
+                                       fat, fm, ri, self.ki_h, self.ki_q, solution_height,
+                                       solution_discharge, self.r_h[nxt], self.r_q[nxt])
+
+
 def butcher_rows(scheme, dt):
     """(factor rows per stage, c) for a scheme: what perform_stage_hydro receives."""
     ai, bi, ci = _tableau(scheme)

@@ -171,6 +171,29 @@ class OceanoOperator:
             _vp(solution_discharge), _vp(next_ri_height), _vp(next_ri_discharge),
             int(write_scratch), _stream()), "oc_dg_perform_stage_hydro")
 
+    def perform_stage_hydro_ark(self, cur_stage, cur_time, factor_residual,
+                                factor_tilde_residual, factor_matrix, current_ri,
+                                vec_ki_height, vec_ki_discharge, solution_height,
+                                solution_discharge, next_ri_height, next_ri_discharge):
+        """The additive (IMEX) stage, OceanDG.h:196-208 with body 1936-2119.
+        factor_matrix is the operator member the integrator sets before every stage
+        (d~_stage dt, AdditiveRungeKuttaIntegrator.h:239, 291); vec_ki_height holds
+        n_stages + 1 vectors, vec_ki_discharge 2 n_stages (explicit and implicit parts)."""
+        n_h, n_q = len(vec_ki_height), len(vec_ki_discharge)
+        fac = np.ascontiguousarray(np.asarray(factor_residual, np.float64)[:cur_stage + 1])
+        fat = np.ascontiguousarray(np.asarray(factor_tilde_residual,
+                                              np.float64)[:cur_stage + 1])
+        kh = (C.c_void_p * n_h)(*[k.data_ptr() if k is not None else None
+                                  for k in vec_ki_height])
+        kq = (C.c_void_p * n_q)(*[k.data_ptr() if k is not None else None
+                                  for k in vec_ki_discharge])
+        vel = current_ri[-1] if len(current_ri) > 2 else None
+        check(lib().oc_dg_perform_stage_hydro_ark(
+            self._h, cur_stage, cur_time, dptr(fac), dptr(fat), cur_stage + 1,
+            float(factor_matrix), _vp(current_ri[0]), _vp(current_ri[1]), _vp(vel), kh, n_h,
+            kq, n_q, _vp(solution_height), _vp(solution_discharge), _vp(next_ri_height),
+            _vp(next_ri_discharge), _stream()), "oc_dg_perform_stage_hydro_ark")
+
     def apply(self, current_time, src, dst):
         """dst = M^-1 L(src); src/dst = [zeta, q] (+ velocity in src when viscous)."""
         vel = src[2] if len(src) > 2 else None

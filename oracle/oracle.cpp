@@ -291,6 +291,37 @@ struct Model
     return s;
   }
 
+  // ShallowWater.h:437-458: the non-stiff part of the source (pressure gradient, wind,
+  // Coriolis); par = {z_b, manning n, tau_x, tau_y, f}
+  V2 source_nonstiff(double z, V2 q, V2 dz, const double *par) const
+  {
+    const double h = depth(z, par[0]);
+    const double mgh = -g * h;
+    V2 s;
+    s.x = mgh * dz.x + par[2] + par[4] * q.y;
+    s.y = mgh * dz.y + par[3] + -par[4] * q.x;
+    return s;
+  }
+
+  // ShallowWater.h:461-472 with BottomFrictionManning::source (BottomFriction.h:186-199)
+  V2 source_stiff(double z, V2 q, double zb, double manning) const
+  {
+    const V2 v = velocity(z, q, zb);
+    const double h = depth(z, zb);
+    const double inv_depth = 1. / (std::exp(std::log(h) * 0.33333333333));
+    const double vnorm = std::sqrt(v.x * v.x + v.y * v.y);
+    const double hcf = g * manning * manning * inv_depth * vnorm;
+    return {-(hcf * v.x), -(hcf * v.y)};
+  }
+
+  // BottomFrictionManning::jacobian (BottomFriction.h:201-212)
+  double friction_jacobian(V2 v, double manning, double h) const
+  {
+    const double inv_depth = 1. / (std::exp(std::log(h) * 1.33333333333));
+    const double vnorm = std::sqrt(v.x * v.x + v.y * v.y);
+    return g * manning * manning * inv_depth * vnorm;
+  }
+
   double riemann_invariant_p(double z, V2 q, V2 n, double zb) const
   {
     const V2 v = velocity(z, q, zb);
@@ -615,8 +646,9 @@ struct Oracle
   }
 
   // ---------------------------- local_apply_cell_discharge (OceanDG.h:696-743) --------
+  // nonstiff = true: local_apply_cell_nonstiff_discharge (OceanDG.h:746-791), friction out
   void cell_discharge(const double *src_h, const double *src_q, const double *src_v,
-                      double *dst) const
+                      double *dst, bool nonstiff = false) const
   {
     const int m = nq, nqp = nq * nq;
     std::vector<double> z(nqp), zx(nqp), zy(nqp), qx(nqp), qy(nqp), uxx(nqp), uxy(nqp),
@@ -649,7 +681,8 @@ struct Oracle
                                       {h * (uyx[q] / hx), h * (uyy[q] / hy)}};
               double F[2][2];
               model.advective_diffusive_flux(zq, qq, G, par[0], F);
-              const V2 S = model.source(zq, qq, dz, par);
+              const V2 S = nonstiff ? model.source_nonstiff(zq, qq, dz, par)
+                                    : model.source(zq, qq, dz, par);
               g0x[q] = F[0][0] / hx * JxW;
               g0y[q] = F[0][1] / hy * JxW;
               g1x[q] = F[1][0] / hx * JxW;
@@ -1023,6 +1056,201 @@ struct Oracle
       }
   }
 
+  // ------------------ local_apply_cell_stiff_discharge (OceanDG.h:794-826) ------------
+  void cell_stiff_discharge(const double *src_h, const double *src_q, double *dst) const
+  {
+    const int m = nq, nqp = nq * nq;
+    std::vector<double> z(nqp), qx(nqp), qy(nqp), vx(nqp), vy(nqp);
+    for (int c = 0; c < N; ++c)
+      {
+        const double hx = geom[4 * c + 2], hy = geom[4 * c + 3];
+        eval_values(src_h + (size_t)c * nd, Sq, z.data());
+        eval_values(src_q + (size_t)c * 2 * nd, Sq, qx.data());
+        eval_values(src_q + (size_t)c * 2 * nd + nd, Sq, qy.data());
+        for (int b = 0; b < m; ++b)
+          for (int a = 0; a < m; ++a)
+            {
+              const int q = a + m * b;
+              const double JxW = gq_w[a] * gq_w[b] * hx * hy;
+              const double *par = &cell0[((size_t)c * nqp + q) * 5];
+              const V2 S = model.source_stiff(z[q], V2{qx[q], qy[q]}, par[0], par[1]);
+              vx[q] = S.x * JxW;
+              vy[q] = S.y * JxW;
+            }
+        integrate_values(vx.data(), Sq, dst + (size_t)c * 2 * nd);
+        integrate_values(vy.data(), Sq, dst + (size_t)c * 2 * nd + nd);
+      }
+  }
+
+  // ------------------ local_apply_cell_mass_discharge (OceanDG.h:858-879) -------------
+  // dst += M src at Gauss(p+1) (exact on the affine cells)
+  void cell_mass_discharge(const double *src, double *dst) const
+  {
+    const int m = n1, nqp = n1 * n1;
+    std::vector<double> v(nqp);
+    for (int c = 0; c < N; ++c)
+      {
+        const double hx = geom[4 * c + 2], hy = geom[4 * c + 3];
+        for (int d = 0; d < 2; ++d)
+          {
+            eval_values(src + ((size_t)c * 2 + d) * nd, Sn, v.data());
+            for (int b = 0; b < m; ++b)
+              for (int a = 0; a < m; ++a)
+                v[a + m * b] *= gn_w[a] * gn_w[b] * hx * hy;
+            integrate_values(v.data(), Sn, dst + ((size_t)c * 2 + d) * nd);
+          }
+      }
+  }
+
+  // ------- local_apply_inverse_modified_mass_matrix_discharge (OceanDG.h:1574-1645) ---
+  // out = (S^T W' S)^-1 in with W' = JxW (1 + fm J(v, n, h)) at Gauss(p+1), the state
+  // (z, q) from ri_h, ri_q and (z_b, n) from cell_2 (FE degree >= 1 branch)
+  void inverse_modified_mass(const double *in, const double *ri_h, const double *ri_q,
+                             double fm, double *out) const
+  {
+    const int nqp = n1 * n1;
+    std::vector<double> z(nqp), qx(nqp), qy(nqp), ijxw(nqp), T(nd), X(nd);
+    for (int c = 0; c < N; ++c)
+      {
+        const double hx = geom[4 * c + 2], hy = geom[4 * c + 3];
+        eval_values(ri_h + (size_t)c * nd, Sn, z.data());
+        eval_values(ri_q + (size_t)c * 2 * nd, Sn, qx.data());
+        eval_values(ri_q + (size_t)c * 2 * nd + nd, Sn, qy.data());
+        for (int b = 0; b < n1; ++b)
+          for (int a = 0; a < n1; ++a)
+            {
+              const int q = a + n1 * b;
+              const double zb = cell2[((size_t)c * nqp + q) * 2];
+              const double cf = cell2[((size_t)c * nqp + q) * 2 + 1];
+              const V2 v = model.velocity(z[q], V2{qx[q], qy[q]}, zb);
+              ijxw[q] = 1. / (gn_w[a] * gn_w[b] * hx * hy);
+              ijxw[q] *= 1. / (1. + fm * model.friction_jacobian(v, cf, model.depth(z[q], zb)));
+            }
+        for (int d = 0; d < 2; ++d)
+          {
+            const double *I = in + ((size_t)c * 2 + d) * nd;
+            double *O = out + ((size_t)c * 2 + d) * nd;
+            for (int j = 0; j < n1; ++j)
+              for (int a = 0; a < n1; ++a)
+                {
+                  double s = 0.0;
+                  for (int i = 0; i < n1; ++i)
+                    s += Sninv(i, a) * I[i + n1 * j];
+                  T[a + n1 * j] = s;
+                }
+            for (int b = 0; b < n1; ++b)
+              for (int a = 0; a < n1; ++a)
+                {
+                  double s = 0.0;
+                  for (int j = 0; j < n1; ++j)
+                    s += Sninv(j, b) * T[a + n1 * j];
+                  X[a + n1 * b] = s * ijxw[a + n1 * b];
+                }
+            for (int b = 0; b < n1; ++b)
+              for (int i = 0; i < n1; ++i)
+                {
+                  double s = 0.0;
+                  for (int a = 0; a < n1; ++a)
+                    s += Sninv(i, a) * X[a + n1 * b];
+                  T[i + n1 * b] = s;
+                }
+            for (int j = 0; j < n1; ++j)
+              for (int i = 0; i < n1; ++i)
+                {
+                  double s = 0.0;
+                  for (int b = 0; b < n1; ++b)
+                    s += Sninv(j, b) * T[i + n1 * b];
+                  O[i + n1 * j] = s;
+                }
+          }
+      }
+  }
+
+  // ------------- perform_stage_hydro, additive (IMEX) RK (OceanDG.h:1936-2119) --------
+  // ki_h: S+1 vectors; ki_q: 2S vectors ([0] front, [2j+1] explicit L_q of stage j,
+  // [2j+2] stiff friction term of stage j < S-1); fm = d_tilde[stage] dt.
+  void perform_stage_hydro_ark(int stage, int n_stages, const double *factor,
+                               const double *factor_tilde, double fm, const double *ri_h,
+                               const double *ri_q, const double *ri_v, double *const *ki_h,
+                               double *const *ki_q, double *w_h, double *w_q, double *r_h,
+                               double *r_q)
+  {
+    const size_t NH = (size_t)N * nd, NQ = (size_t)N * 2 * nd;
+    const bool last = stage == n_stages - 1;
+    last_sweeps = 0;
+    double *Lh = ki_h[stage + 1];
+    std::fill(Lh, Lh + NH, 0.0);
+    cell_height(ri_q, Lh);
+    face_height(ri_h, ri_q, Lh);
+    bdry_face(ri_h, ri_q, Lh, nullptr);
+    double *kex = ki_q[2 * stage + 1];
+    std::fill(kex, kex + NQ, 0.0);
+    cell_discharge(ri_h, ri_q, ri_v, kex, !last);
+    face_discharge(ri_h, ri_q, kex);
+    bdry_face(ri_h, ri_q, nullptr, kex);
+    if (!last)
+      {
+        double *kim = ki_q[2 * stage + 2];
+        std::fill(kim, kim + NQ, 0.0);
+        cell_stiff_discharge(ri_h, ri_q, kim);
+      }
+    for (size_t i = 0; i < NH; ++i)
+      {
+        double k_i = ki_h[1][i];
+        ki_h[0][i] = factor[0] * k_i;
+        for (int j = 1; j < stage + 1; ++j)
+          {
+            k_i = ki_h[j + 1][i];
+            ki_h[0][i] += factor[j] * k_i;
+          }
+      }
+    cell_mass_height(w_h, ki_h[0]);
+    if (last)
+      inverse_mass_height(w_h, ki_h[0]);
+    else
+      {
+        std::memcpy(r_h, w_h, NH * sizeof(double));
+        inverse_mass_height(r_h, ki_h[0]);
+      }
+    if (last)
+      {
+        for (size_t i = 0; i < NQ; ++i)
+          {
+            double kex_i = ki_q[1][i];
+            ki_q[0][i] = factor[0] * kex_i;
+            for (int j = 1; j < stage + 1; ++j)
+              {
+                kex_i = ki_q[2 * j + 1][i];
+                const double kim_i = ki_q[2 * j][i];
+                ki_q[0][i] += factor[j] * kex_i + factor[j - 1] * kim_i;
+              }
+          }
+        inverse_mass_tensor(ki_q[0], r_q, 2);
+        for (size_t i = 0; i < NQ; ++i)
+          {
+            const double sol_i = r_q[i];
+            w_q[i] += sol_i;
+          }
+      }
+    else
+      {
+        for (size_t i = 0; i < NQ; ++i)
+          {
+            double kex_i = ki_q[1][i];
+            double kim_i = ki_q[2][i];
+            ki_q[0][i] = factor[0] * kex_i + factor_tilde[0] * kim_i;
+            for (int j = 1; j < stage + 1; ++j)
+              {
+                kex_i = ki_q[2 * j + 1][i];
+                kim_i = ki_q[2 * j + 2][i];
+                ki_q[0][i] += factor[j] * kex_i + factor_tilde[j] * kim_i;
+              }
+          }
+        cell_mass_discharge(w_q, ki_q[0]);
+        inverse_modified_mass(ki_q[0], ri_h, ri_q, fm, r_q);
+      }
+  }
+
   // ------------------ perform_stage_hydro, explicit RK (OceanDG.h:1782-1912) ----------
   void perform_stage_hydro(int stage, int n_ki, const double *factor, const double *ri_h,
                            const double *ri_q, const double *ri_v, double *const *ki_h,
@@ -1386,6 +1614,18 @@ int ocr_errors(void *h, const double *zeta, const double *q, const double *exact
   err[4] = einf[0];
   err[5] = einf[1];
   err[6] = einf[2];
+  return 0;
+}
+
+int ocr_perform_stage_hydro_ark(void *h, int stage, int n_stages, const double *factor,
+                                const double *factor_tilde, double fm, const double *ri_h,
+                                const double *ri_q, const double *ri_v, double *const *ki_h,
+                                double *const *ki_q, double *w_h, double *w_q, double *r_h,
+                                double *r_q)
+{
+  static_cast<Oracle *>(h)->perform_stage_hydro_ark(stage, n_stages, factor, factor_tilde, fm,
+                                                    ri_h, ri_q, ri_v, ki_h, ki_q, w_h, w_q,
+                                                    r_h, r_q);
   return 0;
 }
 

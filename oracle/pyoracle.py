@@ -51,6 +51,9 @@ def lib():
             "ocr_project_collocation": (C.c_int, [C.c_void_p, C.c_int, PD, PD]),
             "ocr_errors": (C.c_int, [C.c_void_p, PD, PD, PD, PD]),
             "ocr_velocity_field": (C.c_int, [C.c_void_p, PD, PD, PD]),
+            "ocr_perform_stage_hydro_ark": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PD,
+                                                      C.c_double, PD, PD, PD, C.POINTER(PD),
+                                                      C.POINTER(PD), PD, PD, PD, PD]),
             "ocr_transport_speed": (C.c_double, [C.c_void_p, PD, PD]),
             "ocr_volume": (C.c_double, [C.c_void_p, PD]),
         }
@@ -231,3 +234,31 @@ class Oracle:
     def volume(self, zeta):
         """check_mass cell integral (OceanDG.h:2138-2184): integral of the depth"""
         return lib().ocr_volume(self._h, _d(zeta))
+
+    def perform_stage_hydro_ark(self, stage, n_stages, factor, factor_tilde, fm, ri_h, ri_q,
+                                ri_v, ki_h, ki_q, w_h, w_q, r_h, r_q):
+        """OceanDG.h:1936-2119; ki_h has S+1 vectors, ki_q 2S."""
+        fac = np.ascontiguousarray(factor, np.float64)
+        fat = np.ascontiguousarray(factor_tilde, np.float64)
+        ri_v = np.zeros_like(ri_q) if ri_v is None else ri_v
+        kh = (PD * len(ki_h))(*[_d(k) for k in ki_h])
+        kq = (PD * len(ki_q))(*[_d(k) for k in ki_q])
+        self._check(lib().ocr_perform_stage_hydro_ark(
+            self._h, stage, n_stages, _d(fac), _d(fat), fm, _d(ri_h), _d(ri_q), _d(ri_v), kh,
+            kq, _d(w_h), _d(w_q), _d(r_h), _d(r_q)), "perform_stage_hydro_ark")
+
+    def time_step_ark(self, dt, w_h, w_q, ki_h, ki_q, r_h, r_q, v=None,
+                      scheme="stage_3_order_2"):
+        """One IMEX step of AdditiveRungeKuttaIntegrator (AdditiveRungeKuttaIntegrator.h:
+        207-339) with the reference's vec_ri copies."""
+        from oceano_amd.integrator import ark_tableau
+        ai, bi, ci, at, dt_ = ark_tableau(scheme)
+        S = len(bi)
+        for s in range(S):
+            last = s == S - 1
+            fac = np.array(bi if last else ai[s]) * dt
+            fat = np.array(bi if last else at[s]) * dt
+            fm = dt_[s] * dt if s < len(dt_) else 0.0
+            src = (w_h, w_q) if s == 0 else (r_h, r_q)
+            self.perform_stage_hydro_ark(s, S, fac, fat, fm, src[0].copy(), src[1].copy(), v,
+                                         ki_h, ki_q, w_h, w_q, r_h, r_q)

@@ -1,0 +1,167 @@
+"""The additive (IMEX) Runge-Kutta stage on the GPU against the oracle's restatement of
+OceanDG.h:1936-2119 driven as AdditiveRungeKuttaIntegrator.h:207-339 (stage_3_order_2:
+THREE_STAGE_SECOND_ORDER explicit with TRAPEZOIDAL_BDF2 implicit friction)."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import make_pair, rel_l2, to_dev, to_host
+from oceano_amd import problems
+from oceano_amd.integrator import (AdditiveRungeKuttaIntegrator, ExplicitRungeKuttaIntegrator,
+                                   ark_tableau, butcher_rows)
+from oceano_amd.mesh import Mesh
+from oceano_amd.operator import OceanoOperator
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_steps(o, h, q, dt, n, v=None):
+    S = len(ark_tableau()[1])
+    wh, wq = h.copy(), q.copy()
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(2 * S)]
+    r_h, r_q = o.zeros(1), o.zeros(2)
+    for _ in range(n):
+        o.time_step_ark(dt, wh, wq, ki_h, ki_q, r_h, r_q, v=v)
+    return wh, wq
+
+
+def _gpu_steps(op, h, q, dt, n):
+    w_h, w_q = to_dev(op, h, 1), to_dev(op, q, 2)
+    rk = AdditiveRungeKuttaIntegrator()
+    rk.reinit(op)
+    for i in range(n):
+        rk.perform_time_step(op, i * dt, dt, w_h, w_q)
+    torch.cuda.synchronize()
+    return to_host(op, w_h, 1), to_host(op, w_q, 2)
+
+
+def _rough_vortex(degree, manning):
+    """hanging-node vortex over a smooth bed with a spatially varying Manning coefficient"""
+    P = problems.vortex(degree=degree, refine=1, refine_fraction=0.3, seed=6)
+    base = P.problem_data
+
+    def data(xy):
+        d = base(xy)
+        d[:, 0] = 0.05 * np.sin(np.pi * xy[:, 0]) * np.cos(np.pi * xy[:, 1])
+        d[:, 1] = manning * (1.0 + 0.5 * np.cos(2 * np.pi * xy[:, 0]))
+        return d
+    P.problem_data = data
+    return P
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+def test_ark_vortex_matches_oracle(cuda, degree):
+    P = _rough_vortex(degree, 0.1)
+    print("[ark] vortex p=%d cells=%d" % (degree, P.mesh.n_cells), flush=True)
+    op, o, h, q = make_pair(P)
+    dt = 2e-4
+    gh, gq = _gpu_steps(op, h, q, dt, 3)
+    oh, oq = _oracle_steps(o, h, q, dt, 3)
+    eh, eq = rel_l2(gh, oh), rel_l2(gq, oq)
+    print("  3 IMEX steps: zeta %.2e q %.2e" % (eh, eq))
+    assert eh < 1e-12 and eq < 1e-12
+    # friction acted: the IMEX step differs from the frictionless one
+    assert rel_l2(gq, _oracle_steps(make_pair(_rough_vortex(degree, 0.0))[1], h, q, dt,
+                                    3)[1]) > 1e-8
+
+
+def test_ark_channel_matches_oracle(cuda):
+    """inflow / outflow channel over bumps with Manning 0.02 (boundary terms included)"""
+    P = problems.channel(degree=2, nx=50, ny=4)
+    print("[ark] channel cells=%d" % P.mesh.n_cells, flush=True)
+    op, o, h, q = make_pair(P)
+    dt = 2e-3
+    gh, gq = _gpu_steps(op, h, q, dt, 4)
+    oh, oq = _oracle_steps(o, h, q, dt, 4)
+    eh, eq = rel_l2(gh, oh), rel_l2(gq, oq)
+    print("  4 IMEX steps: zeta %.2e q %.2e" % (eh, eq))
+    assert eh < 1e-12 and eq < 1e-12
+
+
+def test_ark_mixed_precision(cuda):
+    """FP32 integrals, FP64 state and FP64 modified-mass update: the state within 1e-6 of
+    FP64, and the error of the free-surface increment (small on the near-steady vortex,
+    so FP32 cancellation shows in it) no worse than that of the explicit mixed step of the
+    same tableau on the same problem."""
+    P = _rough_vortex(3, 0.1)
+    print("[ark] mixed p=3 cells=%d" % P.mesh.n_cells, flush=True)
+    op, o, h, q = make_pair(P, precision="mixed")
+    dt = 2e-4
+    gh, gq = _gpu_steps(op, h, q, dt, 3)
+    oh, oq = _oracle_steps(o, h, q, dt, 3)
+    eh, eq = rel_l2(gh, oh), rel_l2(gq, oq)
+    ei = rel_l2(gh - h, oh - h)
+    rows, _ = butcher_rows("stage_3_order_2", dt)
+    S = len(rows)
+    w_h, w_q = to_dev(op, h, 1), to_dev(op, q, 2)
+    rk = ExplicitRungeKuttaIntegrator("stage_3_order_2")
+    rk.reinit(op)
+    for i in range(3):
+        rk.perform_time_step(op, i * dt, dt, w_h, w_q)
+    torch.cuda.synchronize()
+    xh, xq = h.copy(), q.copy()
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(S + 1)]
+    for _ in range(3):
+        o.time_step(rows, xh, xq, ki_h, ki_q, o.zeros(1), o.zeros(2))
+    ei_x = rel_l2(to_host(op, w_h, 1) - h, xh - h)
+    print("  3 IMEX steps: zeta %.2e q %.2e; zeta increment %.2e (explicit mixed %.2e)"
+          % (eh, eq, ei, ei_x))
+    assert eh < 1e-6 and eq < 1e-6
+    assert ei < 3 * ei_x + 1e-9
+
+
+def test_ark_uniform_flow_closed_form(cuda):
+    """Uniform depth and discharge over a flat periodic bed: the flux terms vanish and
+    every node follows the scalar IMEX recursion for dq/dt = -J(q) q,
+    J = g n^2 |q/H| / H^(4/3) (the same closed form that pins the oracle)."""
+    p, H, Q, n, g = 3, 1.3, np.array([0.9, -0.4]), 0.03, 9.81
+    m = Mesh.rectangle(0, 2, 0, 1, 8, 4, periodic=(True, True))
+    print("[ark] uniform flow p=%d cells=%d" % (p, m.n_cells), flush=True)
+    op = OceanoOperator(m, p, g=g)
+    N = m.n_cells
+    nd = (p + 1) ** 2
+
+    def flat_bed(xy):
+        d = np.zeros((len(xy), 5))
+        d[:, 1] = n
+        return d
+    op.initialize_data_at_quadrature(flat_bed)
+    h = np.full(N * nd, H)
+    q = np.tile(np.repeat(Q, nd), N)
+    dt = 0.05
+    wh, wq = _gpu_steps(op, h, q, dt, 1)
+    a, b, c, at, dtl = ark_tableau()
+
+    def J(r):
+        return g * n * n * np.linalg.norm(r / H) / np.exp(np.log(H) * 1.33333333333)
+    F = lambda r: -J(r) * r
+    r0 = Q
+    r1 = (Q + dt * at[0][0] * F(r0)) / (1 + dt * dtl[0] * J(r0))
+    r2 = (Q + dt * at[1][0] * F(r0) + dt * at[1][1] * F(r1)) / (1 + dt * dtl[1] * J(r1))
+    q_new = Q + dt * (b[0] * F(r0) + b[1] * F(r1) + b[2] * F(r2))
+    wq = wq.reshape(N, 2, nd)
+    print("  max |q - closed form| %.2e %.2e" % (np.abs(wq[:, 0] - q_new[0]).max(),
+                                                 np.abs(wq[:, 1] - q_new[1]).max()))
+    # 1e-12 relative: the collocation inverse of the modified mass at p = 3 loses ~100 ulp
+    tol = 1e-12 * np.linalg.norm(q_new)
+    assert np.abs(wq[:, 0] - q_new[0]).max() < tol
+    assert np.abs(wq[:, 1] - q_new[1]).max() < tol
+    assert np.abs(wh - H).max() < 1e-12 * H
+
+
+def test_ark_argument_errors(cuda):
+    P = problems.vortex(degree=1, refine=1)
+    op, data, w_h, w_q = problems.setup_operator(P)
+    ki_h = [op.initialize_vector(0) for _ in range(4)]
+    ki_q = [op.initialize_vector(1) for _ in range(6)]
+    r_h, r_q = op.initialize_vector(0), op.initialize_vector(1)
+    print("[ark] argument errors", flush=True)
+    with pytest.raises(RuntimeError, match="inconsistent"):
+        op.perform_stage_hydro_ark(0, 0.0, [1e-3], [1e-3], 1e-3, [w_h, w_q], ki_h, ki_q[:4],
+                                   w_h, w_q, r_h, r_q)
+    with pytest.raises(RuntimeError, match="distinct"):
+        op.perform_stage_hydro_ark(1, 0.0, [1e-3, 1e-3], [1e-3, 1e-3], 1e-3, [r_h, r_q],
+                                   ki_h, ki_q, w_h, w_q, r_h, r_q)
+    torch.cuda.synchronize()

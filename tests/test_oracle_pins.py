@@ -333,3 +333,68 @@ def test_volume_is_integral_of_depth():
     # integral over [0,2]x[0,1] of 1 + 0.3x + 0.5 + 0.1x - 0.2y^2
     expect = 2.0 * 1.5 + 0.4 * 2.0 - 0.2 * 2.0 / 3.0
     assert abs(o.volume(h) - expect) < 1e-13
+
+
+# ---------------------------------------------------- IMEX ARK stage (N4) -------------
+def _ark_step(o, h, q, dt, v=None):
+    S = 3
+    w_h, w_q = h.copy(), q.copy()
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(2 * S)]
+    o.time_step_ark(dt, w_h, w_q, ki_h, ki_q, o.zeros(1), o.zeros(2), v=v)
+    return w_h, w_q
+
+
+def test_ark_without_friction_is_the_explicit_scheme():
+    """With zero friction the stiff terms vanish and the modified mass is the mass, so the
+    IMEX step is the explicit stage_3_order_2 step (the momentum update is assembled as
+    M^-1 (M q + sum a_j L_j) instead of q + sum a_j M^-1 L_j: round-off apart)."""
+    P = problems.vortex(degree=2, refine=1, refine_fraction=0.3, seed=2)
+    o = po.Oracle.from_mesh(P.mesh, 2)
+    d = o.initialize_data_at_quadrature(P.problem_data)
+    h, q = o.project_hydro(P.initial, d["cell_2"][:, 0])
+    dt = 4e-4
+    ah, aq = _ark_step(o, h, q, dt)
+    rows, _ = butcher_rows("stage_3_order_2", dt)
+    S = len(rows)
+    eh, eq = h.copy(), q.copy()
+    o.time_step(rows, eh, eq, [o.zeros(1) for _ in range(S + 1)],
+                [o.zeros(2) for _ in range(S + 1)], o.zeros(1), o.zeros(2))
+    assert np.linalg.norm(ah - eh) < 1e-13 * np.linalg.norm(eh)
+    assert np.linalg.norm(aq - eq) < 1e-13 * np.linalg.norm(eq)
+    assert np.linalg.norm(aq - q) > 1e-6 * np.linalg.norm(q)  # the step did something
+
+
+def test_ark_uniform_flow_friction_closed_form():
+    """Uniform depth H and discharge Q over a flat periodic bed: every flux term vanishes
+    and the momentum obeys dq/dt = -J(q) q, J = g n^2 |q/H| / H^(4/3) (Manning).  The IMEX
+    step then reduces to the scalar recursion of the scheme: r_1 = (q + at_00 F0) /
+    (1 + d_0 J0), r_2 = (q + at_10 F0 + at_11 F1) / (1 + d_1 J1), q' = q + sum b_j F_j
+    (F_j = -J(r_j) r_j, dt-scaled coefficients), at every node."""
+    from oceano_amd.integrator import ark_tableau
+    p, H, Q, n, g = 2, 1.3, np.array([0.9, -0.4]), 0.03, 9.81
+    m = Mesh.rectangle(0, 2, 0, 1, 4, 2, periodic=(True, True))
+    o = po.Oracle.from_mesh(m, p, g=g)
+    o.initialize_data_at_quadrature(
+        lambda xy: np.stack([np.zeros(len(xy)), np.full(len(xy), n), np.zeros(len(xy)),
+                             np.zeros(len(xy)), np.zeros(len(xy))], 1))
+    N = o.N * o.nd
+    h = o.project(np.full(N, H), 1)
+    q = o.project(np.ascontiguousarray(np.tile(Q, (N, 1))), 2)
+    dt = 0.05
+    wh, wq = _ark_step(o, h, q, dt)
+
+    def J(r):
+        u = r / H
+        return g * n * n * np.linalg.norm(u) / np.exp(np.log(H) * 1.33333333333)
+    a, b, c, at, dtl = ark_tableau()
+    F = lambda r: -J(r) * r
+    r0 = Q
+    r1 = (Q + dt * at[0][0] * F(r0)) / (1 + dt * dtl[0] * J(r0))
+    r2 = (Q + dt * at[1][0] * F(r0) + dt * at[1][1] * F(r1)) / (1 + dt * dtl[1] * J(r1))
+    q_new = Q + dt * (b[0] * F(r0) + b[1] * F(r1) + b[2] * F(r2))
+    wq = wq.reshape(o.N, 2, o.nd)
+    assert np.abs(wq[:, 0] - q_new[0]).max() < 1e-13
+    assert np.abs(wq[:, 1] - q_new[1]).max() < 1e-13
+    assert np.abs(wh - H).max() < 1e-13
+    assert abs(q_new[0] - Q[0]) > 1e-4  # friction acted
