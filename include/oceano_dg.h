@@ -93,6 +93,7 @@ void oc_mesh_destroy(oc_mesh *mesh);
 /* ------------------------------------------------------------------------------------
  * Stage operator.  Replaces SpaceDiscretization::OceanoOperator (OceanDG.h:157-380).
  * ---------------------------------------------------------------------------------- */
+typedef struct oc_mlp oc_mlp; /* oceano_mlp.h */
 typedef struct oc_dg oc_dg;
 
 typedef struct
@@ -206,6 +207,33 @@ int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
                                   double *const *ki_h, uint32_t n_ki_h, double *const *ki_q,
                                   uint32_t n_ki_q, double *w_h, double *w_q, double *r_h,
                                   double *r_q, void *stream);
+
+/* Learned sub-grid closure (no counterpart in the reference, which has no closure; the
+ * correction is a momentum source, the slot of ShallowWater.h:425-429 used at
+ * OceanDG.h:734-736).  Degrees 1 and 2.
+ *   oc_dg_set_subgrid_stats  host [n_cells][3]: mean, variance, skewness of the sub-grid
+ *                            bed per cell (caller's cell order); also allocates the
+ *                            correction buffer
+ *   oc_dg_set_closure        attach a network (oceano_mlp.h) with 3 nd + 3 inputs (modal
+ *                            coefficients of zeta, hu, hv in the orthonormal tensor
+ *                            Legendre basis of the cell, then the three statistics) and
+ *                            2 nd outputs (modal coefficients of the correction of hu, hv);
+ *                            null detaches.  The handle is not owned and must outlive its
+ *                            use.  With a network attached every explicit stage
+ *                            (oc_dg_perform_stage_hydro, oc_rk_*) adds the correction of
+ *                            its input state to M^-1 L_q; apply, rhs and the additive
+ *                            stage do not.  A cell whose output is not finite gets no
+ *                            correction (the plain sub-grid model).
+ *   oc_dg_closure_features   the network inputs of every cell, device [n_cells][3 nd + 3]
+ *                            FP32, caller's cell order (training data)
+ *   oc_dg_closure_source     the correction of a state, device vector in operator order
+ *                            (2 components) */
+int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats);
+int oc_dg_set_closure(oc_dg *op, const oc_mlp *mlp);
+int oc_dg_closure_features(oc_dg *op, const double *zeta, const double *q, float *features,
+                           void *stream);
+int oc_dg_closure_source(oc_dg *op, const double *zeta, const double *q, double *tau,
+                         void *stream);
 
 /* dst = M^-1 L(src): step-67's OceanoOperator::apply (declared OceanDG.h:178-180; its
  * body in the reference is dead code).  Both fields use the exact tensor-product inverse

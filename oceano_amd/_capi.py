@@ -44,6 +44,7 @@ U32 = C.c_uint32
 U64 = C.c_uint64
 D = C.c_double
 PD = C.POINTER(C.c_double)
+PF = C.POINTER(C.c_float)
 PI32 = C.POINTER(C.c_int32)
 PI64 = C.POINTER(C.c_int64)
 
@@ -109,6 +110,21 @@ SIGNATURES = {
     "oc_dg_compute_errors": (C.c_int, [P, PD, PD, PD, PD]),
     "oc_dg_set_timing": (C.c_int, [P, I32]),
     "oc_dg_timing_report": (C.c_int, [P, PD, PI64]),
+    "oc_dg_set_subgrid_stats": (C.c_int, [P, PD]),
+    "oc_dg_set_closure": (C.c_int, [P, P]),
+    "oc_dg_closure_features": (C.c_int, [P, P, P, P, P]),
+    "oc_dg_closure_source": (C.c_int, [P, P, P, P, P]),
+    "oc_mlp_create": (C.c_int, [I32, I32, U64, C.POINTER(P)]),
+    "oc_mlp_destroy": (None, [P]),
+    "oc_mlp_n_params": (I32, []),
+    "oc_mlp_dims": (C.c_int, [P, PI32, PI32]),
+    "oc_mlp_get_params": (C.c_int, [P, PF]),
+    "oc_mlp_set_params": (C.c_int, [P, PF]),
+    "oc_mlp_set_normalization": (C.c_int, [P, PF, PF, PF, PF]),
+    "oc_mlp_forward": (C.c_int, [P, I64, P, P, P]),
+    "oc_mlp_gradients": (C.c_int, [P, I64, P, P, PF, PD, P]),
+    "oc_mlp_adam": (C.c_int, [P, C.c_float, C.c_float, C.c_float, C.c_float, P]),
+    "oc_mlp_train_step": (C.c_int, [P, I64, P, P, C.c_float, PD, P]),
     "oc_icbc_vortex": (C.c_int, [PD, D, I64, PD, PD]),
     "oc_icbc_channel": (C.c_int, [PD, U64, I64, PD, PD]),
 }
@@ -146,6 +162,11 @@ def check(code, where):
 def dptr(a):
     """ctypes double* of a C-contiguous float64 numpy array."""
     return a.ctypes.data_as(PD)
+
+
+def fptr(a):
+    """ctypes float* of a C-contiguous float32 numpy array."""
+    return a.ctypes.data_as(PF)
 
 
 def iptr(a):

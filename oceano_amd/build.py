@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liboceano_dg.so")
 
-SOURCES = ["mesh.cpp", "icbc.cpp", "dg_operator.hip", "dg_rk.hip",
+SOURCES = ["mesh.cpp", "icbc.cpp", "dg_operator.hip", "dg_rk.hip", "mlp.hip",
            "dg_stage_p1.hip", "dg_stage_p2.hip", "dg_stage_p3.hip", "dg_stage_p4.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -32,6 +32,7 @@ COMMON = [
 def _headers():
     hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
     hs.append(os.path.join(ROOT, "include", "oceano_dg.h"))
+    hs.append(os.path.join(ROOT, "include", "oceano_mlp.h"))
     return max(os.path.getmtime(h) for h in hs)
 
 

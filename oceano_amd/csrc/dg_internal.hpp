@@ -10,6 +10,7 @@
 
 #include "dg_kernels.hpp"
 #include "oceano_dg.h"
+#include "oceano_mlp.h"
 
 namespace oceano {
 void set_error(const std::string &msg);
@@ -54,6 +55,10 @@ struct oc_dg
   int *d_defer = nullptr;   // mixed precision: cells whose FP64 depth solve is deferred
   double *d_post = nullptr; // per-batch results of the reductions [NB]
   double wet_margin = 10.0;
+  // learned closure (oc_dg_set_closure): network (not owned), per-cell sub-grid statistics
+  // [batch][3][lane], modal<->nodal tables [nd][nd], correction buffer [batch][2][nd][lane]
+  const oc_mlp *closure_mlp = nullptr;
+  double *d_cstats = nullptr, *d_cvi = nullptr, *d_cv = nullptr, *d_closure = nullptr;
   unsigned long long *d_count = nullptr;
   std::vector<double> cell2_host;
   std::vector<double> zb_gauss_host;  // z_b at the OC_STORE_CELL_GAUSS points
@@ -93,6 +98,7 @@ struct oc_dg
     o.jmat = d_jmat;
     o.wet_margin = wet_margin;
     o.defer = d_defer;
+    o.closure = nullptr;  // set per stage when a closure network is attached
     return o;
   }
 };
@@ -124,6 +130,8 @@ namespace oceano {
                      double *dst_q, hipStream_t s);                                        \
   int post_d##P(oc_dg *op, const OpArgs &O, int what, const double *zeta, const double *q,  \
                 double *out, hipStream_t s);                                               \
+  int closure_d##P(oc_dg *op, const OpArgs &O, const double *zeta, const double *q,         \
+                   float *features, hipStream_t s);                                        \
   int stage_ark_d##P(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stages,         \
                      const double *factor, const double *factor_tilde, double fm,          \
                      const double *ri_h, const double *ri_q, const double *ri_v,           \

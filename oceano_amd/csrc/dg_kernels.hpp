@@ -81,6 +81,7 @@ struct OpArgs
   const float *jmat;
   double wet_margin;
   int *defer;  // mixed path: per cell slot, 1 = depth solve left to k_depth_deferred
+  const double *closure;  // [batch][2][nd][lane] M^-1-form momentum source, or nullptr
 };
 
 struct StageArgs

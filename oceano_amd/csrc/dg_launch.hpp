@@ -2,6 +2,7 @@
 #pragma once
 
 #include "dg_ark.hpp"
+#include "dg_closure.hpp"
 #include "dg_internal.hpp"
 #include "dg_post.hpp"
 #include "dg_stage.hpp"

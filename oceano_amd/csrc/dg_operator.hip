@@ -16,12 +16,15 @@
 
 #include "dg_internal.hpp"
 #include "oceano_dg.h"
+#include "mlp_core.hpp"
 #include "rules.hpp"
 
 namespace oceano {
 
 thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+constexpr int mlp_inputs_max = mlp::IN;
 
 static int bad_argument(const char *fn)
 {
@@ -150,6 +153,38 @@ std::vector<float> jacobi_matrix(int p, int K)
   for (int i = 0; i < nd * nd; ++i)
     out[i] = float(J[i]);
   return out;
+}
+
+int closure_dispatch(oc_dg *op, const OpArgs &O, const double *zeta, const double *q,
+                     float *features, hipStream_t s)
+{
+  switch (op->P)
+    {
+      case 1:
+        return closure_d1(op, O, zeta, q, features, s);
+      case 2:
+        return closure_d2(op, O, zeta, q, features, s);
+      case 3:
+        return closure_d3(op, O, zeta, q, features, s);
+      default:
+        return closure_d4(op, O, zeta, q, features, s);
+    }
+}
+
+// orthonormal Legendre polynomial of degree n on [0, 1] at t
+double legendre01(int n, double t)
+{
+  const double x = 2.0 * t - 1.0;
+  double a = 1.0, b = x;
+  if (n == 0)
+    return 1.0;
+  for (int k = 1; k < n; ++k)
+    {
+      const double c = ((2 * k + 1) * x * b - k * a) / (k + 1);
+      a = b;
+      b = c;
+    }
+  return std::sqrt(2.0 * n + 1.0) * b;
 }
 
 bool g_tables_ready[64] = {};
@@ -338,6 +373,10 @@ void free_all(oc_dg *op)
   f(op->d_cell1);
   f(op->d_cell2);
   f(op->d_cell2n);
+  f(op->d_cstats);
+  f(op->d_cvi);
+  f(op->d_cv);
+  f(op->d_closure);
   f(op->d_face);
   f(op->d_bz);
   f(op->d_bvals);
@@ -1004,8 +1043,15 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
         set_error("oc_dg_perform_stage_hydro: null ki vector");
         return OC_ERR_ARGUMENT;
       }
-  const OpArgs O = op->args();
+  OpArgs O = op->args();
   hipStream_t s = (hipStream_t)stream;
+  if (op->closure_mlp)
+    {
+      // the learned correction of this stage's input, added after the inverse mass
+      if (const int rc = closure_dispatch(op, O, ri_h, ri_q, nullptr, s))
+        return rc;
+      O.closure = op->d_closure;
+    }
   switch (op->P)
     {
       case 1:
@@ -1342,6 +1388,104 @@ int oc_dg_volume(oc_dg *op, const double *zeta, double *volume, void *stream)
   for (double v : h)
     sum += v;
   *volume = sum;
+  return OC_OK;
+}
+
+int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats)
+{
+  if (!op || (!stats && op->N > 0))
+    return bad_argument(__func__);
+  if (3 * op->nd + 3 > mlp_inputs_max)
+    {
+      set_error("oc_dg_set_subgrid_stats: the closure supports degrees 1 and 2");
+      return OC_ERR_ARGUMENT;
+    }
+  const int n1 = op->n1, nd = op->nd, NB = op->NB;
+  std::vector<double> il((size_t)NB * 3 * CB, 0.0);
+  for (int c = 0; c < op->N; ++c)
+    {
+      const int i = op->pos[c];
+      for (int k = 0; k < 3; ++k)
+        il[((size_t)(i / CB) * 3 + k) * CB + i % CB] = stats[(size_t)c * 3 + k];
+    }
+  int rc = upload(&op->d_cstats, il);
+  if (rc)
+    return rc;
+  // V[k][m] = psi_m(node_k) for k = i + n1 j, m = a + n1 b; Vi = V^-1
+  const Rule1D gl = gauss_lobatto(n1);
+  std::vector<double> V1((size_t)n1 * n1);
+  for (int i = 0; i < n1; ++i)
+    for (int a = 0; a < n1; ++a)
+      V1[i * n1 + a] = legendre01(a, gl.x[i]);
+  const std::vector<double> Vi1 = small_inverse(V1, n1);  // [a][i]
+  std::vector<double> V((size_t)nd * nd), Vi((size_t)nd * nd);
+  for (int j = 0; j < n1; ++j)
+    for (int i = 0; i < n1; ++i)
+      for (int b = 0; b < n1; ++b)
+        for (int a = 0; a < n1; ++a)
+          {
+            const int k = i + n1 * j, m = a + n1 * b;
+            V[(size_t)k * nd + m] = V1[j * n1 + b] * V1[i * n1 + a];
+            Vi[(size_t)m * nd + k] = Vi1[a * n1 + i] * Vi1[b * n1 + j];
+          }
+  if ((rc = upload(&op->d_cv, V)) || (rc = upload(&op->d_cvi, Vi)))
+    return rc;
+  return upload(&op->d_closure, std::vector<double>((size_t)NB * CB * 2 * nd, 0.0));
+}
+
+int oc_dg_set_closure(oc_dg *op, const oc_mlp *mlp)
+{
+  if (!op)
+    return bad_argument(__func__);
+  if (!mlp)
+    {
+      op->closure_mlp = nullptr;
+      return OC_OK;
+    }
+  if (!op->d_cstats)
+    {
+      set_error("oc_dg_set_closure: oc_dg_set_subgrid_stats first");
+      return OC_ERR_ARGUMENT;
+    }
+  int32_t n_in = 0, n_out = 0;
+  oc_mlp_dims(mlp, &n_in, &n_out);
+  if (n_in != 3 * op->nd + 3 || n_out != 2 * op->nd)
+    {
+      set_error("oc_dg_set_closure: the network needs 3 nd + 3 inputs and 2 nd outputs");
+      return OC_ERR_ARGUMENT;
+    }
+  op->closure_mlp = mlp;
+  return OC_OK;
+}
+
+int oc_dg_closure_features(oc_dg *op, const double *zeta, const double *q, float *features,
+                           void *stream)
+{
+  if (!op || !zeta || !q || !features)
+    return bad_argument(__func__);
+  if (!op->d_cstats)
+    {
+      set_error("oc_dg_closure_features: oc_dg_set_subgrid_stats first");
+      return OC_ERR_ARGUMENT;
+    }
+  return closure_dispatch(op, op->args(), zeta, q, features, (hipStream_t)stream);
+}
+
+int oc_dg_closure_source(oc_dg *op, const double *zeta, const double *q, double *tau,
+                         void *stream)
+{
+  if (!op || !zeta || !q || !tau)
+    return bad_argument(__func__);
+  if (!op->closure_mlp)
+    {
+      set_error("oc_dg_closure_source: no closure network attached");
+      return OC_ERR_ARGUMENT;
+    }
+  hipStream_t s = (hipStream_t)stream;
+  if (const int rc = closure_dispatch(op, op->args(), zeta, q, nullptr, s))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(tau, op->d_closure, (size_t)op->NB * CB * 2 * op->nd * sizeof(double),
+                         hipMemcpyDeviceToDevice, s));
   return OC_OK;
 }
 

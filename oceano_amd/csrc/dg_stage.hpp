@@ -782,6 +782,15 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   const R inv_area = R(1) / area;
   tensor_inverse<P, R>(T, Racc[0], inv_area);
   tensor_inverse<P, R>(T, Racc[1], inv_area);
+  if (O.closure)
+    {
+      // learned momentum-source correction, already in M^-1 form (k_closure)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < ND; ++k)
+          Racc[c][k] += R(O.closure[vix(batch, 2, c, ND, k, lane)]);
+    }
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll

@@ -57,4 +57,10 @@ int stage_ark_d3(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stages,
                                       ri_q, ri_v, ki_h, ki_q, w_h, w_q, r_h, r_q, s);
 }
 
+int closure_d3(oc_dg *op, const OpArgs &O, const double *zeta, const double *q,
+                float *features, hipStream_t s)
+{
+  return launch_closure<3>(op, O, zeta, q, features, s);
+}
+
 }  // namespace oceano

@@ -1,0 +1,106 @@
+"""The closure network (include/oceano_mlp.h) seen from Python: BF16 matrix-core training
+and inference kernels of liboceano_dg.so on torch device tensors.
+
+No counterpart in the reference (oceano has no learned closure)."""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import check, fptr, lib
+
+IN, HID, OUT = 32, 64, 32
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class ClosureNetwork:
+    """n_in -> 3 x 64 (GELU) -> n_out; FP32 master weights, BF16 products."""
+
+    def __init__(self, n_in, n_out, seed=0):
+        h = C.c_void_p()
+        check(lib().oc_mlp_create(n_in, n_out, seed, C.byref(h)), "oc_mlp_create")
+        self._h, self.n_in, self.n_out = h, n_in, n_out
+        self.n_params = int(lib().oc_mlp_n_params())
+        self.normalization = (np.zeros(n_in), np.ones(n_in), np.zeros(n_out), np.ones(n_out))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().oc_mlp_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        check(lib().oc_mlp_get_params(self._h, fptr(p)), "oc_mlp_get_params")
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.n_params:
+            raise ValueError("expected %d parameters" % self.n_params)
+        check(lib().oc_mlp_set_params(self._h, fptr(p)), "oc_mlp_set_params")
+
+    def set_normalization(self, x_mean, x_std, y_shift, y_scale):
+        arrs = [np.ascontiguousarray(v, np.float32) for v in (x_mean, x_std, y_shift, y_scale)]
+        check(lib().oc_mlp_set_normalization(self._h, *[fptr(a) for a in arrs]),
+              "oc_mlp_set_normalization")
+        self.normalization = tuple(a.astype(np.float64) for a in arrs)
+
+    @staticmethod
+    def _f32(t):
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError("device float32 contiguous tensors expected")
+        return t
+
+    def forward(self, x):
+        x = self._f32(x)
+        y = torch.empty((x.shape[0], self.n_out), dtype=torch.float32, device=x.device)
+        check(lib().oc_mlp_forward(self._h, x.shape[0], _vp(x), _vp(y), _stream()),
+              "oc_mlp_forward")
+        return y
+
+    def gradients(self, x, t):
+        """(loss, gradient as a host float32 array); synchronises"""
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        check(lib().oc_mlp_gradients(self._h, x.shape[0], _vp(self._f32(x)), _vp(self._f32(t)),
+                                     fptr(g), C.byref(loss), _stream()), "oc_mlp_gradients")
+        return loss.value, g
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        check(lib().oc_mlp_adam(self._h, lr, beta1, beta2, eps, _stream()), "oc_mlp_adam")
+
+    def train_step(self, x, t, lr, want_loss=False):
+        loss = C.c_double()
+        check(lib().oc_mlp_train_step(self._h, x.shape[0], _vp(self._f32(x)),
+                                      _vp(self._f32(t)), lr,
+                                      C.byref(loss) if want_loss else None, _stream()),
+              "oc_mlp_train_step")
+        return loss.value if want_loss else None
+
+    def save(self, path, **extra):
+        np.savez(path, params=self.get_params(), n_in=self.n_in, n_out=self.n_out,
+                 x_mean=self.normalization[0], x_std=self.normalization[1],
+                 y_shift=self.normalization[2], y_scale=self.normalization[3], **extra)
+
+    @classmethod
+    def load(cls, path):
+        d = np.load(path)  # numeric arrays only (allow_pickle stays False)
+        net = cls(int(d["n_in"]), int(d["n_out"]))
+        net.set_params(d["params"])
+        net.set_normalization(d["x_mean"], d["x_std"], d["y_shift"], d["y_scale"])
+        return net

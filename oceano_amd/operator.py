@@ -245,6 +245,33 @@ class OceanoOperator:
               "oc_dg_sum")
         return v.value
 
+    # ------------------------------------------------------ learned closure -------
+    def set_subgrid_stats(self, stats):
+        """[n_cells, 3] sub-grid (mean, variance, skewness) of the bed, cell-major"""
+        v = np.ascontiguousarray(stats, dtype=np.float64)
+        check(lib().oc_dg_set_subgrid_stats(self._h, dptr(v)), "oc_dg_set_subgrid_stats")
+
+    def set_closure(self, network):
+        """attach a ClosureNetwork (None detaches); the operator keeps a reference"""
+        self._closure = network
+        check(lib().oc_dg_set_closure(self._h, network.handle if network else None),
+              "oc_dg_set_closure")
+
+    def closure_features(self, zeta, q):
+        """network inputs [n_cells, 3 nd + 3] (float32, device, cell-major rows)"""
+        x = torch.empty((self.n_cells, 3 * self.dofs_per_cell + 3), dtype=torch.float32,
+                        device=self.device)
+        check(lib().oc_dg_closure_features(self._h, _vp(zeta), _vp(q), _vp(x), _stream()),
+              "oc_dg_closure_features")
+        return x
+
+    def closure_source(self, zeta, q):
+        """the attached network's correction of a state (operator order, 2 components)"""
+        tau = self.initialize_vector(1)
+        check(lib().oc_dg_closure_source(self._h, _vp(zeta), _vp(q), _vp(tau), _stream()),
+              "oc_dg_closure_source")
+        return tau
+
     def count_nonfinite(self, vec):
         n = np.zeros(1, np.int64)
         check(lib().oc_dg_count_nonfinite(self._h, _vp(vec), vec.numel(),
