@@ -212,7 +212,9 @@ int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
  * correction is a momentum source, the slot of ShallowWater.h:425-429 used at
  * OceanDG.h:734-736).  Degrees 1 and 2.
  *   oc_dg_set_subgrid_stats  host [n_cells][3]: mean, variance, skewness of the sub-grid
- *                            bed per cell (caller's cell order); also allocates the
+ *                            bed per cell (caller's cell order); `active` (host
+ *                            [n_cells], null = all) selects the cells the network corrects,
+ *                            the others keep the plain model; also allocates the
  *                            correction buffer
  *   oc_dg_set_closure        attach a network (oceano_mlp.h) with 3 nd + 3 inputs (modal
  *                            coefficients of zeta, hu, hv in the orthonormal tensor
@@ -228,7 +230,7 @@ int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
  *                            FP32, caller's cell order (training data)
  *   oc_dg_closure_source     the correction of a state, device vector in operator order
  *                            (2 components) */
-int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats);
+int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats, const int32_t *active);
 int oc_dg_set_closure(oc_dg *op, const oc_mlp *mlp);
 int oc_dg_closure_features(oc_dg *op, const double *zeta, const double *q, float *features,
                            void *stream);

@@ -38,6 +38,10 @@ int oc_mlp_set_params(oc_mlp *mlp, const float *params);
 /* host arrays: x_mean, x_std [n_in]; y_shift, y_scale [n_out] (std and scale > 0) */
 int oc_mlp_set_normalization(oc_mlp *mlp, const float *x_mean, const float *x_std,
                              const float *y_shift, const float *y_scale);
+/* host arrays lo, hi [n_out] (either may be null = unbounded): inference saturates the
+ * normalised output to [lo, hi] before de-normalising (the trained range); training is
+ * unaffected.  Default unbounded. */
+int oc_mlp_set_output_range(oc_mlp *mlp, const float *lo, const float *hi);
 /* y = network(x) */
 int oc_mlp_forward(oc_mlp *mlp, int64_t n, const float *x, float *y, void *stream);
 /* gradient of the loss over the batch; grad (host, n_params floats) and loss (host) may be

@@ -110,7 +110,7 @@ SIGNATURES = {
     "oc_dg_compute_errors": (C.c_int, [P, PD, PD, PD, PD]),
     "oc_dg_set_timing": (C.c_int, [P, I32]),
     "oc_dg_timing_report": (C.c_int, [P, PD, PI64]),
-    "oc_dg_set_subgrid_stats": (C.c_int, [P, PD]),
+    "oc_dg_set_subgrid_stats": (C.c_int, [P, PD, PI32]),
     "oc_dg_set_closure": (C.c_int, [P, P]),
     "oc_dg_closure_features": (C.c_int, [P, P, P, P, P]),
     "oc_dg_closure_source": (C.c_int, [P, P, P, P, P]),
@@ -121,6 +121,7 @@ SIGNATURES = {
     "oc_mlp_get_params": (C.c_int, [P, PF]),
     "oc_mlp_set_params": (C.c_int, [P, PF]),
     "oc_mlp_set_normalization": (C.c_int, [P, PF, PF, PF, PF]),
+    "oc_mlp_set_output_range": (C.c_int, [P, PF, PF]),
     "oc_mlp_forward": (C.c_int, [P, I64, P, P, P]),
     "oc_mlp_gradients": (C.c_int, [P, I64, P, P, PF, PD, P]),
     "oc_mlp_adam": (C.c_int, [P, C.c_float, C.c_float, C.c_float, C.c_float, P]),

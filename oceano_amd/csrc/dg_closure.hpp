@@ -6,7 +6,8 @@
 // correction tau of hu and hv, which is brought back to nodal values and added to
 // M^-1 L_q by the discharge update (a source s in L_q enters as M s, so adding tau after
 // the inverse mass is the source term of ShallowWater.h:425-429 with s = tau).  A cell
-// whose network output is not finite gets tau = 0: the plain sub-grid model.
+// that is not active (no sub-grid bathymetry nearby) or whose network output is not finite
+// gets tau = 0: the plain sub-grid model.
 #pragma once
 
 #include "dg_internal.hpp"
@@ -21,7 +22,7 @@ namespace oceano {
 struct ClosureArgs
 {
   const double *zeta, *q;
-  const double *stats;  // [batch][3][lane]
+  const double *stats;  // [batch][4][lane]: mean, variance, skewness, active flag
   const double *vi;     // [nd][nd] modal <- nodal
   const double *v;      // [nd][nd] nodal <- modal
   const float *p;       // network parameters (biases)
@@ -65,7 +66,7 @@ __device__ inline void closure_features(const ClosureArgs &C, int batch, int lan
     }
 #pragma unroll
   for (int s = 0; s < 3; ++s)
-    f[3 * ND + s] = float(C.stats[((size_t)batch * 3 + s) * CB + lane]);
+    f[3 * ND + s] = float(C.stats[((size_t)batch * 4 + s) * CB + lane]);
 #pragma unroll
   for (int i = 3 * ND + 3; i < mlp::IN; ++i)
     f[i] = 0.f;
@@ -115,11 +116,11 @@ __global__ __launch_bounds__(CB) void k_closure(OpArgs O, ClosureArgs C)
     forward_tile(s_x + rt * 16 * IN, s_wt, C.p, s_h[0], s_h[1], s_h[2], nullptr,
                  s_y + rt * 16 * OUT);
   double tm[2 * ND];
-  bool finite = active;
+  bool finite = active && C.stats[((size_t)batch * 4 + 3) * CB + lane] != 0.0;
 #pragma unroll
   for (int o = 0; o < 2 * ND; ++o)
     {
-      const float y = s_y[lane * OUT + o] * C.norm[2 * IN + o] + C.norm[2 * IN + OUT + o];
+      const float y = output_value(s_y[lane * OUT + o], C.norm, o);
       finite = finite && isfinite(y);
       tm[o] = double(y);
     }

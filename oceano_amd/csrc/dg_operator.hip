@@ -1391,7 +1391,7 @@ int oc_dg_volume(oc_dg *op, const double *zeta, double *volume, void *stream)
   return OC_OK;
 }
 
-int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats)
+int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats, const int32_t *active)
 {
   if (!op || (!stats && op->N > 0))
     return bad_argument(__func__);
@@ -1401,12 +1401,14 @@ int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats)
       return OC_ERR_ARGUMENT;
     }
   const int n1 = op->n1, nd = op->nd, NB = op->NB;
-  std::vector<double> il((size_t)NB * 3 * CB, 0.0);
+  // [batch][4][lane]: mean, variance, skewness, active flag
+  std::vector<double> il((size_t)NB * 4 * CB, 0.0);
   for (int c = 0; c < op->N; ++c)
     {
       const int i = op->pos[c];
       for (int k = 0; k < 3; ++k)
-        il[((size_t)(i / CB) * 3 + k) * CB + i % CB] = stats[(size_t)c * 3 + k];
+        il[((size_t)(i / CB) * 4 + k) * CB + i % CB] = stats[(size_t)c * 3 + k];
+      il[((size_t)(i / CB) * 4 + 3) * CB + i % CB] = (!active || active[c]) ? 1.0 : 0.0;
     }
   int rc = upload(&op->d_cstats, il);
   if (rc)

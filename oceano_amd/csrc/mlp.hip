@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ p
           const int s = e / OUT, o = e % OUT;
           const int64_t row = base + s;
           if (row < n && o < n_out)
-            Y[row * n_out + o] = s_y[w][e] * norm[2 * IN + o] + norm[2 * IN + OUT + o];
+            Y[row * n_out + o] = output_value(s_y[w][e], norm, o);
         }
       __syncthreads();
     }
@@ -365,6 +365,20 @@ int upload_params(oc_mlp *m, const std::vector<float> &p)
   MLP_TRY(hipMemcpy(m->wt, wt.data(), NT * sizeof(bf16), hipMemcpyHostToDevice));
   MLP_TRY(hipMemcpy(m->wb, wb.data(), NPARAM * sizeof(bf16), hipMemcpyHostToDevice));
   return OC_OK;
+}
+
+std::vector<float> default_norm()
+{
+  std::vector<float> nrm(NNORM, 0.f);
+  for (int i = 0; i < IN; ++i)
+    nrm[IN + i] = 1.f;
+  for (int o = 0; o < OUT; ++o)
+    {
+      nrm[2 * IN + o] = 1.f;
+      nrm[NLO + o] = -INFINITY;
+      nrm[NHI + o] = INFINITY;
+    }
+  return nrm;
 }
 
 void release(oc_mlp *m)
@@ -433,11 +447,7 @@ int oc_mlp_create(int32_t n_in, int32_t n_out, uint64_t seed, oc_mlp **out)
   fill(W2, HID, HID, HID, HID);
   fill(W3, HID, HID, HID, HID);
   fill(W4, HID, OUT, HID, n_out);
-  std::vector<float> nrm(NNORM, 0.f);
-  for (int i = 0; i < IN; ++i)
-    nrm[IN + i] = 1.f;
-  for (int o = 0; o < OUT; ++o)
-    nrm[2 * IN + o] = 1.f;
+  std::vector<float> nrm = default_norm();
   int rc = upload_params(m, p);
   if (!rc && hipMemcpy(m->norm, nrm.data(), NNORM * sizeof(float), hipMemcpyHostToDevice) !=
                hipSuccess)
@@ -501,11 +511,10 @@ int oc_mlp_set_normalization(oc_mlp *m, const float *x_mean, const float *x_std,
       set_error("oc_mlp_set_normalization: null argument");
       return OC_ERR_ARGUMENT;
     }
-  std::vector<float> nrm(NNORM, 0.f);
-  for (int i = 0; i < IN; ++i)
-    nrm[IN + i] = 1.f;
-  for (int o = 0; o < OUT; ++o)
-    nrm[2 * IN + o] = 1.f;
+  std::vector<float> nrm = default_norm();
+  // keep the output range of an earlier oc_mlp_set_output_range
+  MLP_TRY(hipMemcpy(nrm.data() + NLO, m->norm + NLO, 2 * OUT * sizeof(float),
+                    hipMemcpyDeviceToHost));
   for (int i = 0; i < m->n_in; ++i)
     {
       if (!(x_std[i] > 0.f))
@@ -527,6 +536,28 @@ int oc_mlp_set_normalization(oc_mlp *m, const float *x_mean, const float *x_std,
       nrm[2 * IN + OUT + o] = y_shift[o];
     }
   MLP_TRY(hipMemcpy(m->norm, nrm.data(), NNORM * sizeof(float), hipMemcpyHostToDevice));
+  return OC_OK;
+}
+
+int oc_mlp_set_output_range(oc_mlp *m, const float *lo, const float *hi)
+{
+  if (!m)
+    {
+      set_error("oc_mlp_set_output_range: null argument");
+      return OC_ERR_ARGUMENT;
+    }
+  std::vector<float> r(2 * OUT);
+  for (int o = 0; o < OUT; ++o)
+    {
+      r[o] = (lo && o < m->n_out) ? lo[o] : -INFINITY;
+      r[OUT + o] = (hi && o < m->n_out) ? hi[o] : INFINITY;
+      if (!(r[o] <= r[OUT + o]))
+        {
+          set_error("oc_mlp_set_output_range: lo > hi");
+          return OC_ERR_ARGUMENT;
+        }
+    }
+  MLP_TRY(hipMemcpy(m->norm + NLO, r.data(), 2 * OUT * sizeof(float), hipMemcpyHostToDevice));
   return OC_OK;
 }
 

@@ -27,8 +27,17 @@ constexpr int W1 = 0, B1 = W1 + IN * HID, W2 = B1 + HID, B2 = W2 + HID * HID, W3
 // BF16 transposed weights [fan_out][fan_in]: the B operand of the forward products
 constexpr int T1 = 0, T2 = T1 + HID * IN, T3 = T2 + HID * HID, T4 = T3 + HID * HID,
               NT = T4 + OUT * HID;
-// normalisation block: mu[IN], inv_sigma[IN], scale[OUT], shift[OUT]
-constexpr int NNORM = 2 * IN + 2 * OUT;
+// normalisation block: mu[IN], inv_sigma[IN], scale[OUT], shift[OUT], lo[OUT], hi[OUT]
+// (inference saturates the normalised output to [lo, hi]; training does not)
+constexpr int NNORM = 2 * IN + 4 * OUT;
+constexpr int NLO = 2 * IN + 2 * OUT, NHI = 2 * IN + 3 * OUT;
+
+// de-normalised network output of column o at inference
+__device__ inline float output_value(float yn, const float *norm, int o)
+{
+  yn = fminf(fmaxf(yn, norm[NLO + o]), norm[NHI + o]);
+  return yn * norm[2 * IN + o] + norm[2 * IN + OUT + o];
+}
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));

@@ -30,6 +30,7 @@ class ClosureNetwork:
         self._h, self.n_in, self.n_out = h, n_in, n_out
         self.n_params = int(lib().oc_mlp_n_params())
         self.normalization = (np.zeros(n_in), np.ones(n_in), np.zeros(n_out), np.ones(n_out))
+        self.output_range = (np.full(n_out, -np.inf), np.full(n_out, np.inf))
 
     def __del__(self):
         try:
@@ -59,6 +60,13 @@ class ClosureNetwork:
         check(lib().oc_mlp_set_normalization(self._h, *[fptr(a) for a in arrs]),
               "oc_mlp_set_normalization")
         self.normalization = tuple(a.astype(np.float64) for a in arrs)
+
+    def set_output_range(self, lo, hi):
+        """saturation of the normalised output at inference (the trained range)"""
+        a = np.ascontiguousarray(lo, np.float32)
+        b = np.ascontiguousarray(hi, np.float32)
+        check(lib().oc_mlp_set_output_range(self._h, fptr(a), fptr(b)), "oc_mlp_set_output_range")
+        self.output_range = (a.astype(np.float64), b.astype(np.float64))
 
     @staticmethod
     def _f32(t):
@@ -95,7 +103,8 @@ class ClosureNetwork:
     def save(self, path, **extra):
         np.savez(path, params=self.get_params(), n_in=self.n_in, n_out=self.n_out,
                  x_mean=self.normalization[0], x_std=self.normalization[1],
-                 y_shift=self.normalization[2], y_scale=self.normalization[3], **extra)
+                 y_shift=self.normalization[2], y_scale=self.normalization[3],
+                 y_lo=self.output_range[0], y_hi=self.output_range[1], **extra)
 
     @classmethod
     def load(cls, path):
@@ -103,4 +112,6 @@ class ClosureNetwork:
         net = cls(int(d["n_in"]), int(d["n_out"]))
         net.set_params(d["params"])
         net.set_normalization(d["x_mean"], d["x_std"], d["y_shift"], d["y_scale"])
+        if "y_lo" in d:
+            net.set_output_range(d["y_lo"], d["y_hi"])
         return net

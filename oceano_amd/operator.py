@@ -246,10 +246,14 @@ class OceanoOperator:
         return v.value
 
     # ------------------------------------------------------ learned closure -------
-    def set_subgrid_stats(self, stats):
-        """[n_cells, 3] sub-grid (mean, variance, skewness) of the bed, cell-major"""
+    def set_subgrid_stats(self, stats, active=None):
+        """[n_cells, 3] sub-grid (mean, variance, skewness) of the bed, cell-major;
+        active [n_cells] (None = all): the cells a closure network corrects"""
         v = np.ascontiguousarray(stats, dtype=np.float64)
-        check(lib().oc_dg_set_subgrid_stats(self._h, dptr(v)), "oc_dg_set_subgrid_stats")
+        a = None if active is None else np.ascontiguousarray(active, dtype=np.int32)
+        check(lib().oc_dg_set_subgrid_stats(self._h, dptr(v),
+                                            None if a is None else _capi.iptr(a)),
+              "oc_dg_set_subgrid_stats")
 
     def set_closure(self, network):
         """attach a ClosureNetwork (None detaches); the operator keeps a reference"""

@@ -93,8 +93,11 @@ class Network:
                 y = z
         return (y, cache) if keep else y
 
-    def predict(self, x):
+    def predict(self, x, lo=None, hi=None):
+        """de-normalised output; lo / hi saturate the normalised output (inference)"""
         y = self.forward(x)[:, :self.n_out]
+        if lo is not None:
+            y = np.clip(y, lo, hi)
         return y * self.y_scale + self.y_shift
 
     def loss_and_gradient(self, x, t):
