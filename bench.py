@@ -63,8 +63,10 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
-def algorithmic_bytes(op, stage, static):
-    """Compulsory HBM bytes of one stage (FP64 vectors, int32 topology), per kernel.
+def algorithmic_bytes(op, stage, static, k_bytes=8):
+    """Compulsory HBM bytes of one stage (FP64 state vectors, k vectors of k_bytes per
+    value -- 4 on the mixed path, whose k values are FP32 numbers --, int32 topology),
+    per kernel.
 
     stage (one pass): reads zeta_r, q_r, k_h[1..stage], k_q[1..stage], w_h, w_q; writes
     k_h[stage+1], k_q[stage+1], r_h, r_q (w_h, w_q at the last stage); plus per cell the
@@ -76,7 +78,8 @@ def algorithmic_bytes(op, stage, static):
     topo = (4 + 8 + 8) * 4 * n
     geom = 16 * n
     stat = sum(static.values())
-    one_pass = (v1 + v2) * (1 + stage + 1 + 2) + topo + geom + stat
+    kf = k_bytes / 8.0
+    one_pass = (v1 + v2) * (1 + 2) + (v1 + v2) * kf * (stage + 1) + topo + geom + stat
     rhs = v1 + v2 + v1 + topo + geom
     dis = v1 + v2 + v2 + stage * v2 + 2 * v2 + topo + geom
     jac = (stage + 1) * v1 + 2 * v1 + geom
@@ -128,6 +131,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slot-chunk", type=int, default=None,
                     help="library option slot_chunk (cell grouping by face signature)")
+    ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
+                    help="time the work-layout options per operator and keep the fastest "
+                         "(auto: on for the ensemble)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this size 3 times (a kernel of "
                          "known HBM traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -151,10 +157,23 @@ def main():
     multi = len(groups) > 1
     n_steps_timed = args.steps if not multi else S * (-(-args.steps // S))
 
+    tune_on = args.autotune == "on" or (args.autotune == "auto" and multi)
+    tuned = {}
+
     def run(precision):
+        from oceano_amd import tune
         G = []
-        for P, dt in zip(groups, dts):
+        for gi, (P, dt) in enumerate(zip(groups, dts)):
+            if tune_on:
+                chosen, table = tune.autotune(
+                    P, precision, dt=None if isinstance(dt, tuple) else dt,
+                    cfl=dt[1] if isinstance(dt, tuple) else None)
+                tuned.setdefault(precision, []).append(
+                    {"options": chosen, "us_per_step": {json.dumps(o): t * 1e6
+                                                        for o, t in table}})
             op, data, w_h, w_q = problems.setup_operator(P, precision=precision)
+            if tune_on:
+                tune.apply_options(tune.LIBRARY_DEFAULTS)
             if isinstance(dt, tuple):  # main.cc:1503-1505: dt = CFL / transport speed
                 dt = dt[1] / float("%.3g" % op.transport_speed(w_h, w_q))
             rows, _ = butcher_rows("stage_4_order_4", dt)
@@ -227,7 +246,8 @@ def main():
             rep = op.timing_report()
             op.set_timing(False)
             finish_step(g, n_steps_timed)
-            per_stage = [algorithmic_bytes(op, s, {}) for s in range(S)]
+            per_stage = [algorithmic_bytes(op, s, {}, 4 if precision == "mixed" else 8)
+                         for s in range(S)]
             for name, (ms, n) in rep.items():
                 if n == 0:
                     continue
@@ -295,7 +315,8 @@ def main():
                    "dofs": dofs, "degree": degree_label, "operators": len(groups),
                    "scheme": "classic RK4 (Butcher), stage = step",
                    "dt": [g["dt"] for g in head["groups"]] if multi else head["groups"][0]["dt"],
-                   "precision": head_p, "memory": note},
+                   "precision": head_p, "memory": note,
+                   "autotuned": tuned.get(head_p) if tune_on else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,

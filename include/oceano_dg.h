@@ -44,7 +44,10 @@ const char *oc_last_error(void);
  *                 two subfaces or on the boundary) within chunks of this many 64-cell
  *                 batches of the caller's order; 0 (default) = one XCD's share of the
  *                 batches for OC_PRECISION_FP64 and the caller's order for the mixed
- *                 path; < 0 = keep the caller's order.  Results do not depend on it. */
+ *                 path; < 0 = keep the caller's order.  Results do not depend on it.
+ *   "xcd_swizzle" 1 (default): workgroups are dealt so that each of the 8 XCDs runs a
+ *                 contiguous range of batches (neighbour traces stay in its L2); 0: the
+ *                 hardware's round-robin order.  Results do not depend on it. */
 int oc_set_option(const char *name, int64_t value);
 
 /* ------------------------------------------------------------------------------------
@@ -177,7 +180,10 @@ int oc_dg_unpack(oc_dg *op, int32_t n_components, const double *op_order, double
  * ki_h / ki_q are host arrays of n_ki device pointers; n_ki = n_stages + 1 and the last
  * stage is stage == n_ki - 2 (OceanDG.h:1827).  The scratch entries ki_h[0], ki_q[0] are
  * written only if `write_scratch` is nonzero (the time integrator never reads them).
- * ri_vel is read only when the viscosity is nonzero (it multiplies the only use). */
+ * ri_vel is read only when the viscosity is nonzero (it multiplies the only use).
+ * OC_PRECISION_MIXED: the k vectors ki_h[1..], ki_q[1..] are operator-internal and hold
+ * FP32 values in the first half of each vector (the FP32 integrals' values, stored
+ * exactly); every combination with them stays FP64.  The scratch entries stay FP64. */
 int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const double *factor,
                               uint32_t n_factor, const double *ri_h, const double *ri_q,
                               const double *ri_vel, double *const *ki_h, double *const *ki_q,
@@ -199,7 +205,8 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
  *                           ri_q: the stage reads the input after writing r_h)
  * The last stage updates w_h, w_q in place and writes r_q = M^-1 (b . kex + b . kim).
  * Boundary data come from the stores set with oc_dg_set_boundary_values, so `time` is not
- * read.  ri_vel is read only when the viscosity is nonzero. */
+ * read.  ri_vel is read only when the viscosity is nonzero.  OC_PRECISION_MIXED: ki_h[1..]
+ * hold FP32 values in their first half as in the explicit stage; ki_q stay FP64. */
 int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
                                   const double *factor, const double *factor_tilde,
                                   uint32_t n_factor, double factor_matrix, const double *ri_h,

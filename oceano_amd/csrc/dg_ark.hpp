@@ -12,7 +12,9 @@
 //                  (b_j kex_j + b_{j-1} kim_j), r_q = M^-1 front, w_q += r_q (2040-2071).
 // The height part of the stage and the explicit momentum integrals reuse the ERK kernels
 // (k_height_rhs, k_discharge with the friction switched off, k_height_jacobi).  FP64 for
-// the update; the integrals follow the operator's precision.
+// the update; the integrals follow the operator's precision.  The height k vectors are
+// stored in the integral precision (as in the explicit stage), the discharge k vectors in
+// FP64 (the stiff terms and the modified-mass update are FP64).
 #pragma once
 
 #include "dg_stage.hpp"
@@ -324,7 +326,7 @@ int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stag
   const dim3 grid(op->NB), block(CB);
   hipEvent_t ev = nullptr;
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_height_rhs<P, false, R>), grid, block, 0, s, O, ri_h, ri_q,
+  hipLaunchKernelGGL((k_height_rhs<P, false, R, R>), grid, block, 0, s, O, ri_h, ri_q,
                      ki_h[stage + 1]);
   timed_end(op, s, TK_HEIGHT_RHS, ev);
   // explicit momentum integrals: the full operator at the last stage, without the friction
@@ -357,7 +359,7 @@ int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stag
   J.rk_dst = last ? w_h : r_h;
   J.sweeps_blk = op->d_sweeps_blk;
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_height_jacobi<P, R>), grid, block, 0, s, O, J);
+  hipLaunchKernelGGL((k_height_jacobi<P, R, R>), grid, block, 0, s, O, J);
   timed_end(op, s, TK_HEIGHT_JACOBI, ev);
   // momentum update
   ArkArgs A{};

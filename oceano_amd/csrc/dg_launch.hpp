@@ -20,15 +20,17 @@ struct RkArgs
   double *dst;
 };
 
-// dst = w + sum_j factor_j k_j, in the reference's order (OceanDG.h:1878-1909)
+// dst = w + sum_j factor_j k_j, in the reference's order (OceanDG.h:1878-1909); the k
+// vectors are stored in KT
+template <typename KT>
 __global__ void k_rk_update(RkArgs A)
 {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < A.n;
        i += (size_t)gridDim.x * blockDim.x)
     {
-      double acc = A.w[i] + A.factor[0] * A.k[0][i];
+      double acc = A.w[i] + A.factor[0] * double(reinterpret_cast<const KT *>(A.k[0])[i]);
       for (int j = 1; j < A.nk; ++j)
-        acc += A.factor[j] * A.k[j][i];
+        acc += A.factor[j] * double(reinterpret_cast<const KT *>(A.k[j])[i]);
       A.dst[i] = acc;
     }
 }
@@ -103,15 +105,15 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
     }
   // split path: every kernel finishes reading ri before the next one overwrites it
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_height_rhs<P, false, R>), grid, block, 0, s, O, ri_h, ri_q,
+  hipLaunchKernelGGL((k_height_rhs<P, false, R, R>), grid, block, 0, s, O, ri_h, ri_q,
                      ki_h[stage + 1]);
   timed_end(op, s, TK_HEIGHT_RHS, ev);
   timed_begin(op, s, &ev);
   StageArgs A = Q;
   if (visc)
-    hipLaunchKernelGGL((k_discharge<P, true, 1, R>), grid, block, 0, s, O, A);
+    hipLaunchKernelGGL((k_discharge<P, true, 3, R>), grid, block, 0, s, O, A);
   else
-    hipLaunchKernelGGL((k_discharge<P, false, 1, R>), grid, block, 0, s, O, A);
+    hipLaunchKernelGGL((k_discharge<P, false, 3, R>), grid, block, 0, s, O, A);
   timed_end(op, s, TK_DISCHARGE, ev);
   RkArgs U{};
   U.n = (size_t)op->NB * CB * 2 * op->nd;
@@ -124,13 +126,13 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
   U.w = w_q;
   U.dst = q_dst;
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL(k_rk_update, dim3((U.n + 255) / 256), dim3(256), 0, s, U);
+  hipLaunchKernelGGL(k_rk_update<R>, dim3((U.n + 255) / 256), dim3(256), 0, s, U);
   timed_end(op, s, TK_RK_UPDATE, ev);
   StageArgs J = H;
   for (uint32_t j = 0; j <= stage; ++j)
     J.kprev[j] = ki_h[j + 1];
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_height_jacobi<P, R>), grid, block, 0, s, O, J);
+  hipLaunchKernelGGL((k_height_jacobi<P, R, R>), grid, block, 0, s, O, J);
   timed_end(op, s, TK_HEIGHT_JACOBI, ev);
   HIP_TRY(hipGetLastError());
   return OC_OK;

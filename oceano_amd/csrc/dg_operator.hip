@@ -435,6 +435,7 @@ bool uniform(const std::vector<double> &v, double &val)
 // Default (0): FP64 sorts within one XCD's share (fewer divergent face passes pay off at one
 // wave per SIMD); the mixed path keeps the caller's order (its two waves per SIMD hide the
 // divergence and it gains more from neighbour locality) -- measured on the C2 vortex.
+int g_xcd_swizzle = 1;  // batch -> workgroup order: each XCD gets a contiguous batch range
 int g_slot_chunk = 0;  // batches per chunk; 0: by precision as above; < 0: the caller's order
 
 int cell_slots(const oc_mesh_desc *mesh, int precision, std::vector<int32_t> &order,
@@ -498,6 +499,11 @@ int oc_set_option(const char *name, int64_t value)
       g_slot_chunk = (int)value;
       return OC_OK;
     }
+  if (std::strcmp(name, "xcd_swizzle") == 0)
+    {
+      g_xcd_swizzle = value != 0;
+      return OC_OK;
+    }
   set_error(std::string("oc_set_option: unknown option ") + name);
   return OC_ERR_ARGUMENT;
 }
@@ -527,6 +533,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
     }
   int rc = OC_OK;
   oc_dg *op = new oc_dg;
+  op->xcd_swizzle = g_xcd_swizzle;
   op->P = prm->degree;
   op->n1 = op->P + 1;
   op->nd = op->n1 * op->n1;

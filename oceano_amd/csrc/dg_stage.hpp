@@ -496,7 +496,8 @@ __device__ __forceinline__ Phys<R> physics(const OpArgs &O)
 }
 
 // L_zeta (INVERT: M^-1 L_zeta, for apply())
-template <int P, bool INVERT, typename R>
+// KT: storage type of dst (a stage's k vector is stored in R, a caller vector in FP64)
+template <int P, bool INVERT, typename R, typename KT = double>
 __global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__restrict__ src_h,
                                                    const double *__restrict__ src_q,
                                                    double *__restrict__ dst)
@@ -519,12 +520,17 @@ __global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__res
     tensor_inverse<P, R>(T, Racc[0], R(1) / (hx * hy));
 #pragma unroll
   for (int k = 0; k < ND; ++k)
-    dst[vix(batch, 1, 0, ND, k, lane)] = double(Racc[0][k]);
+    reinterpret_cast<KT *>(dst)[vix(batch, 1, 0, ND, k, lane)] = KT(Racc[0][k]);
 }
 
 // ---------------------------------------------------------------------------------------
-// L_q of one batch; MODE 0: stage (M^-1, RK update), 1: apply (M^-1 -> dst), 2: rhs
-// The batch's own zeta, q (and velocity) are staged in LDS, lane-interleaved.
+// L_q of one batch; MODE 0: stage (M^-1, RK update), 1: apply (M^-1 -> dst), 2: rhs,
+// 3: stage of the split path (M^-1 -> k vector).  The batch's own zeta, q (and velocity)
+// are staged in LDS, lane-interleaved.
+//
+// k vectors of a stage (ki_h[1..], ki_q[1..]) are stored in the integral precision R: the
+// FP32 path's k values are FP32 numbers, so FP32 storage is exact and halves their
+// traffic; every combination with them (the RK update, the front) is FP64.
 // ---------------------------------------------------------------------------------------
 template <int P, bool VISC>
 constexpr int discharge_lds_words()
@@ -537,6 +543,9 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
                                                 R *lds)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
+  // dst of a stage is a k vector (stored in R); of apply / rhs a caller vector (FP64)
+  using DT = typename std::conditional<MODE == 0 || MODE == 3, R, double>::type;
+  DT *const dst = reinterpret_cast<DT *>(S.dst);
   const Basis<R> &T = basis<R>(P);
   const int lane = threadIdx.x;
   const double *__restrict__ src_h = S.src_h;
@@ -767,7 +776,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int k = 0; k < ND; ++k)
-          S.dst[vix(batch, 2, c, ND, k, lane)] = double(Racc[c][k]);
+          dst[vix(batch, 2, c, ND, k, lane)] = DT(Racc[c][k]);
       return;
     }
   if (MODE == 0 && S.scratch)
@@ -795,7 +804,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   for (int c = 0; c < 2; ++c)
 #pragma unroll
     for (int k = 0; k < ND; ++k)
-      S.dst[vix(batch, 2, c, ND, k, lane)] = double(Racc[c][k]);
+      dst[vix(batch, 2, c, ND, k, lane)] = DT(Racc[c][k]);
   if (MODE == 0)
     {
       // post-op of OceanDG.h:1878-1909: dst = w + sum_j a_j k_{j+1}, k_{stage+1} = Racc,
@@ -814,12 +823,12 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
           issue_barrier();
           for (int j = 0; j < S.stage; ++j)
             {
-              const double *kp = S.kprev[j] + base;
+              const R *kp = reinterpret_cast<const R *>(S.kprev[j]) + base;
               const double fj = S.factor[j];
               double t[CH];
 #pragma unroll
               for (int i = 0; i < CH; ++i)
-                t[i] = kp[(size_t)i * CB];
+                t[i] = double(kp[(size_t)i * CB]);
               issue_barrier();
 #pragma unroll
               for (int i = 0; i < CH; ++i)
@@ -1167,9 +1176,9 @@ __device__ __forceinline__ void store_sweeps(unsigned int *blk, int batch, int k
     blk[batch] = v;
 }
 
-// front from ki_h[1..stage+1] (all in memory) and the Jacobi solve: the path used when
-// the stage's outputs alias its inputs
-template <int P, typename R>
+// front from ki_h[1..stage+1] (all in memory, stored in KT) and the Jacobi solve: the
+// path used when the stage's outputs alias its inputs, and by the additive stage
+template <int P, typename R, typename KT>
 __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
 {
   constexpr int ND = (P + 1) * (P + 1);
@@ -1186,19 +1195,20 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
       const double area = hx * hy;
       double F[ND], Z[ND];
       const size_t base = (size_t)batch * ND * CB + lane;
+      const KT *k0 = reinterpret_cast<const KT *>(S.kprev[0]) + base;
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         {
-          F[i] = S.factor[0] * S.kprev[0][base + (size_t)i * CB];
+          F[i] = S.factor[0] * double(k0[(size_t)i * CB]);
           Z[i] = S.wsol[base + (size_t)i * CB];
         }
       for (int j = 1; j <= S.stage; ++j)
         {
-          const double *kp = S.kprev[j] + base;
+          const KT *kp = reinterpret_cast<const KT *>(S.kprev[j]) + base;
           const double fj = S.factor[j];
 #pragma unroll
           for (int i = 0; i < ND; ++i)
-            F[i] += fj * kp[(size_t)i * CB];
+            F[i] += fj * double(kp[(size_t)i * CB]);
         }
       if (centred<R>())
         k = mixed_depth_solve<P, false>(O, T, M, batch, lane, area, F, Z);
@@ -1241,19 +1251,21 @@ __device__ __forceinline__ void depth_deferred_batch(const OpArgs &O, const Stag
       const double area = hx * hy;
       double F[ND], Z[ND];
       const size_t base = (size_t)batch * ND * CB + lane;
+      // k vectors of the FP32 stage are stored in FP32
+      const float *k0 = reinterpret_cast<const float *>(S.kprev[0]) + base;
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         {
-          F[i] = S.factor[0] * S.kprev[0][base + (size_t)i * CB];
+          F[i] = S.factor[0] * double(k0[(size_t)i * CB]);
           Z[i] = S.wsol[base + (size_t)i * CB];
         }
       for (int j = 1; j <= S.stage; ++j)
         {
-          const double *kp = S.kprev[j] + base;
+          const float *kp = reinterpret_cast<const float *>(S.kprev[j]) + base;
           const double fj = S.factor[j];
 #pragma unroll
           for (int i = 0; i < ND; ++i)
-            F[i] += fj * kp[(size_t)i * CB];
+            F[i] += fj * double(kp[(size_t)i * CB]);
         }
       unsigned long long bits = 0;
       double hdummy;
@@ -1331,7 +1343,7 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
         const size_t base = (size_t)batch * ND * CB + lane;
 #pragma unroll
         for (int i = 0; i < ND; ++i)
-          S.dst[base + (size_t)i * CB] = double(Racc[0][i]);  // ki_h[stage+1]
+          reinterpret_cast<R *>(S.dst)[base + (size_t)i * CB] = Racc[0][i];  // ki_h[stage+1]
         if (S.stage == 0)
           {
 #pragma unroll
@@ -1340,21 +1352,22 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
           }
         else
           {
+            const R *k0 = reinterpret_cast<const R *>(S.kprev[0]) + base;
 #pragma unroll
             for (int i = 0; i < ND; ++i)
-              F[i] = S.kprev[0][base + (size_t)i * CB];
+              F[i] = double(k0[(size_t)i * CB]);
             issue_barrier();
 #pragma unroll
             for (int i = 0; i < ND; ++i)
               F[i] = S.factor[0] * F[i];
             for (int j = 1; j < S.stage; ++j)
               {
-                const double *kp = S.kprev[j] + base;
+                const R *kp = reinterpret_cast<const R *>(S.kprev[j]) + base;
                 const double fj = S.factor[j];
                 double t[ND];
 #pragma unroll
                 for (int i = 0; i < ND; ++i)
-                  t[i] = kp[(size_t)i * CB];
+                  t[i] = double(kp[(size_t)i * CB]);
                 issue_barrier();
 #pragma unroll
                 for (int i = 0; i < ND; ++i)
