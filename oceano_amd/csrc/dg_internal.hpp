@@ -52,7 +52,7 @@ struct oc_dg
   double *d_bvals = nullptr;
   unsigned int *d_sweeps_blk = nullptr;
   float *d_jmat = nullptr;  // mixed precision: closed-form Jacobi matrix (unit cell)
-  int *d_defer = nullptr;   // mixed precision: cells whose FP64 depth solve is deferred
+  unsigned long long *d_defer = nullptr;  // mixed precision: per batch, deferred lanes
   double *d_post = nullptr; // per-batch results of the reductions [NB]
   double wet_margin = 10.0;
   // learned closure (oc_dg_set_closure): network (not owned), per-cell sub-grid statistics

@@ -80,7 +80,7 @@ struct OpArgs
   // [nd][nd] row-major (delta = J r / area), and the wet-cell safety factor
   const float *jmat;
   double wet_margin;
-  int *defer;  // mixed path: per cell slot, 1 = depth solve left to k_depth_deferred
+  unsigned long long *defer;  // mixed path: per batch, lanes whose depth solve is deferred
   const double *closure;  // [batch][2][nd][lane] M^-1-form momentum source, or nullptr
 };
 

@@ -709,7 +709,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       (rc = upload(&op->d_fid, fid)) || (rc = upload(&op->d_finfo, finfo)) ||
       (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps_blk, (size_t)NB)) ||
       (rc = upload(&op->d_jmat, jacobi_matrix(op->P, op->max_it))) ||
-      (rc = dalloc(&op->d_defer, (size_t)NB * CB)) || (rc = dalloc(&op->d_post, (size_t)NB)) ||
+      (rc = dalloc(&op->d_defer, (size_t)NB)) || (rc = dalloc(&op->d_post, (size_t)NB)) ||
       (rc = dalloc(&op->d_count, 1)))
     {
       free_all(op);

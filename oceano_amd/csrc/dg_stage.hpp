@@ -1082,8 +1082,9 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
 // delta^K = sum_{k<K} (I - D^-1 M)^k D^-1 r with r = sum_j a_j L_zeta,j, a fixed matrix
 // (on the unit cell) times r / area.  Cells near the floor take the FP64 sweep.
 // Returns the sweep count reported for the cell (max_it on the closed form).  DEFER: a
-// cell that needs the FP64 sweep is left unchanged and flagged in O.defer for
-// k_depth_deferred (keeps the FP64 sweep's registers out of the fused FP32 stage kernel).
+// cell that needs the FP64 sweep is left unchanged and flagged in the batch's mask
+// O.defer[batch] for k_depth_deferred (keeps the FP64 sweep's registers out of the fused
+// FP32 stage kernel).  Called by the active lanes; lane 0 of a batch is always active.
 template <int P, bool DEFER>
 __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTables &T,
                                                  const Phys<double> &M, int batch, int lane,
@@ -1143,7 +1144,11 @@ __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTa
     }
   const bool closed = hmin > float(O.wet_margin) * dmax && hmin > 1e-6f;
   if (DEFER)
-    O.defer[batch * CB + lane] = closed ? 0 : 1;
+    {
+      const unsigned long long m = __ballot(!closed);
+      if (lane == 0)
+        O.defer[batch] = m;
+    }
   if (closed)
     {
 #pragma unroll
@@ -1289,27 +1294,22 @@ __device__ __forceinline__ void depth_deferred_batch(const OpArgs &O, const Stag
 // The FP64 depth solve of the cells the fused mixed-precision stage flagged (near the
 // depth floor): front from ki_h[1..stage+1] (all written by then) and zeta^n = w_h (the
 // stage leaves these cells' w_h untouched); adds their sweeps to the batch count.  One
-// wave looks after DEFER_BATCHES consecutive batches, so that a stage without flagged
-// cells (the common case) costs a short launch.
-constexpr int DEFER_BATCHES = 4;
+// wave gathers the masks of 64 batches in one load and visits only the flagged ones, so
+// a stage without flagged cells (the common case) costs a launch of n_batches / 64 waves.
+constexpr int DEFER_BATCHES = 64;
 template <int P>
 __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 {
-  // the flags of all the wave's batches in one group of loads
-  bool mine[DEFER_BATCHES];
-#pragma unroll
-  for (int i = 0; i < DEFER_BATCHES; ++i)
+  const int lane = threadIdx.x;
+  const int b0 = blockIdx.x * DEFER_BATCHES;
+  const unsigned long long m = b0 + lane < O.n_batches ? O.defer[b0 + lane] : 0ull;
+  unsigned long long todo = __ballot(m != 0ull);
+  while (todo)
     {
-      const int cell = (blockIdx.x * DEFER_BATCHES + i) * CB + threadIdx.x;
-      mine[i] = cell < O.n_cells && O.defer[cell] != 0;
-    }
-  issue_barrier();
-  for (int i = 0; i < DEFER_BATCHES; ++i)
-    {
-      const int batch = blockIdx.x * DEFER_BATCHES + i;
-      if (batch >= O.n_batches)
-        break;
-      depth_deferred_batch<P>(O, S, batch, mine[i]);
+      const int j = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const unsigned long long mj = __shfl(m, j);
+      depth_deferred_batch<P>(O, S, b0 + j, ((mj >> lane) & 1ull) != 0);
     }
 }
 
