@@ -88,7 +88,7 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
         hipLaunchKernelGGL((k_stage<P, false, R>), dim3(2 * op->NB), block, 0, s, O, H, Q);
       timed_end(op, s, TK_STAGE, ev);
       HIP_TRY(hipGetLastError());
-      if (std::is_same<R, float>::value)
+      if (std::is_same<R, float>::value && defer_depth_solve(P))
         {
           // FP64 depth solve of the cells the FP32 stage left near the depth floor; a wave
           // without such a cell returns at once

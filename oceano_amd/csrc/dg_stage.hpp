@@ -1085,6 +1085,11 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
 // cell that needs the FP64 sweep is left unchanged and flagged in the batch's mask
 // O.defer[batch] for k_depth_deferred (keeps the FP64 sweep's registers out of the fused
 // FP32 stage kernel).  Called by the active lanes; lane 0 of a batch is always active.
+// The FP32 stage leaves the FP64 depth solve of cells near the floor to k_depth_deferred
+// at p >= 3, where its registers would cut the stage kernel's occupancy; at p <= 2 the
+// stage kernel does it itself (same occupancy, one launch less per stage).
+__host__ __device__ constexpr bool defer_depth_solve(int p) { return p >= 3; }
+
 template <int P, bool DEFER>
 __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTables &T,
                                                  const Phys<double> &M, int batch, int lane,
@@ -1386,7 +1391,7 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
         Z[i] = S.wsol[vix(batch, 1, 0, ND, i, lane)];
       issue_barrier();
       if (centred<R>())
-        k = mixed_depth_solve<P, true>(O, T, M, batch, lane, area, F, Z);
+        k = mixed_depth_solve<P, defer_depth_solve(P)>(O, T, M, batch, lane, area, F, Z);
       else
         {
           unsigned long long bits = 0;
