@@ -522,7 +522,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--noise", type=float, default=0.02)
     ap.add_argument("--time", type=float, default=2.0)
-    ap.add_argument("--out", default="gpurun_out/closure")
+    ap.add_argument("--out", default="closure_run")
     a = ap.parse_args(argv)
     import os
     os.makedirs(a.out, exist_ok=True)

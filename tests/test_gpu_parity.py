@@ -1,6 +1,6 @@
 """GPU (liboceano_dg.so, through the C ABI) vs the CPU oracle, FP64.
 
-Bar: relative L2 <= 1e-12 per vector (BASELINE's FP64 parity target), on conforming,
+Bar: relative L2 <= 1e-12 per vector (the FP64 parity target), on conforming,
 hanging-node and boundary-condition meshes for every degree 1..4.
 """
 import numpy as np
