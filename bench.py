@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slot-chunk", type=int, default=None,
                     help="library option slot_chunk (cell grouping by face signature)")
+    ap.add_argument("--ark", action="store_true",
+                    help="also time the additive (IMEX) stage on the same workload")
     ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
                     help="time the work-layout options per operator and keep the fastest "
                          "(auto: on for the ensemble)")
@@ -352,11 +354,46 @@ def main():
             # per component, relative to the norm of the whole discharge (hv ~ 0 in a channel)
             "rel_l2_hu": float(np.linalg.norm(comp(ms, 0) - comp(fs, 0)) / qn),
             "rel_l2_hv": float(np.linalg.norm(comp(ms, 1) - comp(fs, 1)) / qn)}
+    if args.ark and not multi:
+        line["ark"] = {p: run_ark(groups[0], dts[0], p, n_steps_timed) for p in precisions}
     if not args.no_cpu and not multi:
         line["cpu_baseline"] = cpu_baseline(groups[0], args.cpu_stages)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+
+
+def run_ark(problem, dt, precision, n_stages):
+    """The additive (IMEX) stage_3_order_2 scheme on the same workload: eager stages (five
+    kernels each: L_zeta, explicit L_q, stiff source, depth solve, modified-mass update)
+    timed between two synchronisations; DoF-stages/s and the per-kernel event times."""
+    import torch
+    from oceano_amd import problems
+    from oceano_amd.integrator import AdditiveRungeKuttaIntegrator
+    op, data, w_h, w_q = problems.setup_operator(problem, precision=precision)
+    if isinstance(dt, tuple):
+        dt = dt[1] / float("%.3g" % op.transport_speed(w_h, w_q))
+    rk = AdditiveRungeKuttaIntegrator()
+    rk.reinit(op)
+    S = rk.n_stages()
+    steps = max(1, n_stages // S)
+    for i in range(2):
+        rk.perform_time_step(op, i * dt, dt, w_h, w_q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        rk.perform_time_step(op, i * dt, dt, w_h, w_q)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    op.set_timing(True)
+    rk.perform_time_step(op, 0.0, dt, w_h, w_q)
+    torch.cuda.synchronize()
+    rep = op.timing_report()
+    op.set_timing(False)
+    return {"value": problem.dofs() * steps * S / el, "ms_per_stage": el / (steps * S) * 1e3,
+            "stages": steps * S, "dt": dt, "nonfinite": op.count_nonfinite(w_h) +
+            op.count_nonfinite(w_q),
+            "kernel_ms_per_stage": {k: v[0] / S for k, v in rep.items() if v[1]}}
 
 
 if __name__ == "__main__":
