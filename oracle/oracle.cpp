@@ -438,6 +438,9 @@ struct Oracle
   std::vector<double> bvals;  // [n_bdry][nq][3] boundary function values (set per stage)
   int lane_width = 1;
   int max_iteration_height = 15;
+  // threads > 1: the loops run on OpenMP threads (CPU-baseline timing only; the face
+  // scatter then accumulates atomically, so sums are not in a fixed order)
+  int threads = 1;
   long last_sweeps = 0;
 
   Oracle(int degree) : p(degree)
@@ -617,7 +620,13 @@ struct Oracle
         double s = 0.0;
         for (int a = 0; a < M.r; ++a)
           s += M(a, k) * v[a];
-        R[face_dof(f, k)] += s;
+        if (threads > 1)
+          {
+#pragma omp atomic
+            R[face_dof(f, k)] += s;
+          }
+        else
+          R[face_dof(f, k)] += s;
       }
   }
 
@@ -625,7 +634,10 @@ struct Oracle
   void cell_height(const double *src_q, double *dst) const
   {
     const int m = nq, nqp = nq * nq;
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
     std::vector<double> qx(nqp), qy(nqp), fx(nqp), fy(nqp);
+#pragma omp for schedule(static)
     for (int c = 0; c < N; ++c)
       {
         const double hx = geom[4 * c + 2], hy = geom[4 * c + 3];
@@ -643,6 +655,7 @@ struct Oracle
             }
         integrate_ref_gradients(fx.data(), fy.data(), dst + (size_t)c * nd);
       }
+    }
   }
 
   // ---------------------------- local_apply_cell_discharge (OceanDG.h:696-743) --------
@@ -651,9 +664,12 @@ struct Oracle
                       double *dst, bool nonstiff = false) const
   {
     const int m = nq, nqp = nq * nq;
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
     std::vector<double> z(nqp), zx(nqp), zy(nqp), qx(nqp), qy(nqp), uxx(nqp), uxy(nqp),
       uyx(nqp), uyy(nqp);
     std::vector<double> vx(nqp), vy(nqp), g0x(nqp), g0y(nqp), g1x(nqp), g1y(nqp);
+#pragma omp for schedule(static)
     for (int c = 0; c < N; ++c)
       {
         const double hx = geom[4 * c + 2], hy = geom[4 * c + 3];
@@ -696,6 +712,7 @@ struct Oracle
         integrate_values(vy.data(), Sq, R + nd);
         integrate_ref_gradients(g1x.data(), g1y.data(), R + nd);
       }
+    }
   }
 
   // geometry of an inner face: JxW factor (face length of the finer side)
@@ -725,7 +742,10 @@ struct Oracle
   // ---------------------------- local_apply_face_height (OceanDG.h:937-982) -----------
   void face_height(const double *src_h, const double *src_q, double *dst) const
   {
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
     std::vector<double> zm(ng), qxm(ng), qym(ng), zp(ng), qxp(ng), qyp(ng), vm(ng), vp(ng);
+#pragma omp for schedule(static)
     for (size_t fi = 0; fi < inner.size(); ++fi)
       {
         const InnerFace &F = inner[fi];
@@ -746,13 +766,17 @@ struct Oracle
         face_integrate(vp.data(), F.fp, F.sp >= 0 ? Sgsub[F.sp] : Sg, dst + (size_t)F.cp * nd);
         face_integrate(vm.data(), F.fm, F.sm >= 0 ? Sgsub[F.sm] : Sg, dst + (size_t)F.cm * nd);
       }
+    }
   }
 
   // ---------------------------- local_apply_face_discharge (OceanDG.h:985-1050) -------
   void face_discharge(const double *src_h, const double *src_q, double *dst) const
   {
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
     std::vector<double> zm(ng), qxm(ng), qym(ng), zp(ng), qxp(ng), qyp(ng);
     std::vector<double> vmx(ng), vmy(ng), vpx(ng), vpy(ng);
+#pragma omp for schedule(static)
     for (size_t fi = 0; fi < inner.size(); ++fi)
       {
         const InnerFace &F = inner[fi];
@@ -785,6 +809,7 @@ struct Oracle
         face_integrate(vmx.data(), F.fm, Mm, dst + (size_t)F.cm * 2 * nd);
         face_integrate(vmy.data(), F.fm, Mm, dst + (size_t)F.cm * 2 * nd + nd);
       }
+    }
   }
 
   // exterior state on a boundary face (OceanDG.h:1130-1203 / 1246-1317)
@@ -844,7 +869,10 @@ struct Oracle
   // ------------------ local_apply_boundary_face_height / _discharge ------------------
   void bdry_face(const double *src_h, const double *src_q, double *dst_h, double *dst_q) const
   {
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
     std::vector<double> zm(nq), qxm(nq), qym(nq), vh(nq), vx(nq), vy(nq);
+#pragma omp for schedule(static)
     for (size_t bi = 0; bi < bdry.size(); ++bi)
       {
         const BdryFace &B = bdry[bi];
@@ -889,6 +917,7 @@ struct Oracle
             face_integrate(vy.data(), f, Sq, dst_q + (size_t)c * 2 * nd + nd);
           }
       }
+    }
   }
 
   // L_zeta and L_q (the two MatrixFree::loop calls of OceanDG.h:1805-1823, dst zeroed)
@@ -909,7 +938,10 @@ struct Oracle
   void cell_mass_height(const double *src, double *dst) const
   {
     const int m = ng, nqp = ng * ng;
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
     std::vector<double> z(nqp), v(nqp);
+#pragma omp for schedule(static)
     for (int c = 0; c < N; ++c)
       {
         const double hx = geom[4 * c + 2], hy = geom[4 * c + 3];
@@ -923,6 +955,7 @@ struct Oracle
             }
         integrate_values(v.data(), Sg, dst + (size_t)c * nd);
       }
+    }
   }
 
   // ---------- local_apply_inverse_mass_matrix_height (OceanDG.h:1374-1507) ------------
@@ -933,9 +966,15 @@ struct Oracle
   {
     const int m = ng, nqp = ng * ng;
     const int W = lane_width;
+    const int n_batches = (N + W - 1) / W;
+    long sweeps = 0;
+#pragma omp parallel num_threads(threads) if (threads > 1) reduction(+ : sweeps)
+    {
     std::vector<double> z(nqp), v(nqp), rhs(nd), resid(nd), D(nd), dz(nd);
-    for (int c0 = 0; c0 < N; c0 += W)
+#pragma omp for schedule(static)
+    for (int bb = 0; bb < n_batches; ++bb)
       {
+        const int c0 = bb * W;
         const int nl = std::min(W, N - c0);
         // dry lift per lane (OceanDG.h:1399-1422)
         for (int l = 0; l < nl; ++l)
@@ -961,7 +1000,7 @@ struct Oracle
         double norm = 1.;
         for (int k = 0; (k < max_iteration_height) && (norm > 1e-12); ++k)
           {
-            ++last_sweeps;
+            ++sweeps;
             norm = 0.;
             for (int l = 0; l < nl; ++l)
               {
@@ -1003,6 +1042,8 @@ struct Oracle
               }
           }
       }
+    }
+    last_sweeps += sweeps;
   }
 
   // ---- local_apply_inverse_mass_matrix_discharge (OceanDG.h:1533-1571): deal.II -------
@@ -1010,7 +1051,10 @@ struct Oracle
   // GL-nodal basis at the Gauss(p+1) points and W = JxW there (exact on affine cells).
   void inverse_mass_tensor(const double *in, double *out, int n_comp) const
   {
+#pragma omp parallel num_threads(threads) if (threads > 1)
+    {
     std::vector<double> T(nd), X(nd);
+#pragma omp for schedule(static)
     for (int c = 0; c < N; ++c)
       {
         const double hx = geom[4 * c + 2], hy = geom[4 * c + 3];
@@ -1054,6 +1098,7 @@ struct Oracle
                 }
           }
       }
+    }
   }
 
   // ------------------ local_apply_cell_stiff_discharge (OceanDG.h:794-826) ------------
@@ -1262,6 +1307,7 @@ struct Oracle
     last_sweeps = 0;
     rhs(ri_h, ri_q, ri_v, ki_h[stage + 1], ki_q[0]);
     // pre-op: front = sum_j factor[j] * ki_h[j+1]  (OceanDG.h:1835-1847)
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
     for (size_t i = 0; i < NH; ++i)
       {
         double k_i = ki_h[1][i];
@@ -1283,6 +1329,7 @@ struct Oracle
     inverse_mass_tensor(ki_q[0], ki_q[stage + 1], 2);
     // post-op (OceanDG.h:1878-1909)
     double *dst = (stage == n_stages - 1) ? w_q : r_q;
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
     for (size_t i = 0; i < NQ; ++i)
       {
         double k_i = ki_q[1][i];
@@ -1409,6 +1456,8 @@ int ocr_set_static(void *h, int store, const double *values)
 }
 
 void ocr_set_lane_width(void *h, int w) { static_cast<Oracle *>(h)->lane_width = w; }
+// loops on n OpenMP threads (n > 1: face sums accumulate atomically; CPU-baseline timing)
+void ocr_set_threads(void *h, int n) { static_cast<Oracle *>(h)->threads = n < 1 ? 1 : n; }
 void ocr_set_max_iterations(void *h, int k)
 {
   static_cast<Oracle *>(h)->max_iteration_height = k;

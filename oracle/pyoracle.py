@@ -39,6 +39,7 @@ def lib():
             "ocr_destroy": (None, [C.c_void_p]),
             "ocr_set_static": (C.c_int, [C.c_void_p, C.c_int, PD]),
             "ocr_set_lane_width": (None, [C.c_void_p, C.c_int]),
+            "ocr_set_threads": (None, [C.c_void_p, C.c_int]),
             "ocr_set_max_iterations": (None, [C.c_void_p, C.c_int]),
             "ocr_last_sweeps": (C.c_long, [C.c_void_p]),
             "ocr_rhs": (C.c_int, [C.c_void_p, PD, PD, PD, PD, PD]),
@@ -153,6 +154,11 @@ class Oracle:
 
     def set_lane_width(self, w):
         lib().ocr_set_lane_width(self._h, w)
+
+    def set_threads(self, n):
+        """n > 1: OpenMP loops (face sums accumulate atomically, rounding-level changes);
+        for CPU-baseline timing only"""
+        lib().ocr_set_threads(self._h, n)
 
     def set_max_iterations(self, k):
         lib().ocr_set_max_iterations(self._h, k)

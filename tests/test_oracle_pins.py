@@ -398,3 +398,23 @@ def test_ark_uniform_flow_friction_closed_form():
     assert np.abs(wq[:, 1] - q_new[1]).max() < 1e-13
     assert np.abs(wh - H).max() < 1e-13
     assert abs(q_new[0] - Q[0]) > 1e-4  # friction acted
+
+
+def test_threaded_oracle_matches_sequential():
+    """the OpenMP loops of the CPU baseline give the sequential stage to rounding (face
+    contributions accumulate in another order)"""
+    from oceano_amd.integrator import butcher_rows
+    P = problems.vortex(degree=2, refine=1, refine_fraction=0.3, seed=3)
+    out = []
+    for threads in (1, 4):
+        o = po.Oracle.from_mesh(P.mesh, 2)
+        d = o.initialize_data_at_quadrature(P.problem_data)
+        h, q = o.project_hydro(P.initial, d["cell_2"][:, 0])
+        o.set_threads(threads)
+        rows, _ = butcher_rows("stage_4_order_4", 2e-4)
+        S = len(rows)
+        o.time_step(rows, h, q, [o.zeros(1) for _ in range(S + 1)],
+                    [o.zeros(2) for _ in range(S + 1)], o.zeros(1), o.zeros(2))
+        out.append((h, q))
+    assert np.linalg.norm(out[1][0] - out[0][0]) < 1e-14 * np.linalg.norm(out[0][0])
+    assert np.linalg.norm(out[1][1] - out[0][1]) < 1e-14 * np.linalg.norm(out[0][1])
