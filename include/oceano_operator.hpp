@@ -187,6 +187,39 @@ public:
           "oc_dg_perform_stage_hydro");
   }
 
+  // OceanDG.h:196-208: the TIMEINTEGRATOR_ADDITIVERUNGEKUTTA overload; reads the public
+  // member factor_matrix that the integrator sets before every stage
+  void perform_stage_hydro(const unsigned int cur_stage, const double cur_time,
+                           const double *factor_residual, const double *factor_tilde_residual,
+                           const std::vector<VectorType> &current_ri,
+                           std::vector<VectorType> &vec_ki_height,
+                           std::vector<VectorType> &vec_ki_discharge,
+                           VectorType &solution_height, VectorType &solution_discharge,
+                           VectorType &next_ri_height, VectorType &next_ri_discharge) const
+  {
+    const size_t n_h = vec_ki_height.size(), n_q = vec_ki_discharge.size();
+    if (current_ri.size() < 2)
+      throw Error(OC_ERR_ARGUMENT, "perform_stage_hydro");
+    std::vector<double *> kh(n_h), kq(n_q);
+    for (size_t j = 0; j < n_h; ++j)
+      kh[j] = vec_ki_height[j].get_values();
+    for (size_t j = 0; j < n_q; ++j)
+      kq[j] = vec_ki_discharge[j].get_values();
+    const double *vel = current_ri.size() > 3 && current_ri.back().size()
+                          ? current_ri.back().get_values()
+                          : nullptr;
+    check(oc_dg_perform_stage_hydro_ark(
+            op_, cur_stage, cur_time, factor_residual, factor_tilde_residual, cur_stage + 1,
+            factor_matrix, current_ri[0].get_values(), current_ri[1].get_values(), vel,
+            kh.data(), (uint32_t)n_h, kq.data(), (uint32_t)n_q, solution_height.get_values(),
+            solution_discharge.get_values(), next_ri_height.get_values(),
+            next_ri_discharge.get_values(), stream_),
+          "oc_dg_perform_stage_hydro_ark");
+  }
+
+  // d~_stage dt of the implicit stage, set by AdditiveRungeKuttaIntegrator (OceanDG.h:163)
+  double factor_matrix = 0.0;
+
   // OceanDG.h:178-180 with dst = M^-1 L(src) for both fields
   void apply(const double current_time, const VectorType &src_height,
              const VectorType &src_discharge, VectorType &dst_height,
@@ -301,6 +334,87 @@ public:
 private:
   std::vector<std::vector<double>> ai;
   std::vector<double> bi, ci;
+};
+
+// TimeIntegrator::AdditiveRungeKuttaIntegrator (AdditiveRungeKuttaIntegrator.h:77-339):
+// stage_3_order_2 = explicit THREE_STAGE_SECOND_ORDER (time_stepping.h:223-245) with the
+// implicit TRAPEZOIDAL_BDF2 (time_stepping.h:340-379); vec_ri copies as in the reference.
+// The reference reads d~[S-1] at the last stage, one past the two coefficients of the
+// implicit tableau; the last stage has no implicit solve, so 0 is used there.
+class AdditiveRungeKuttaIntegrator
+{
+public:
+  enum Scheme
+  {
+    stage_3_order_2
+  };
+
+  explicit AdditiveRungeKuttaIntegrator(Scheme = stage_3_order_2)
+  {
+    const double chi = 2.0 - std::sqrt(2.0), a32 = 0.5;
+    ai = {{chi}, {1.0 - a32, a32}};
+    bi = {0.5 - chi / 4.0, 0.5 - chi / 4.0, chi / 2.0};
+    ci = {0.0, chi, 1.0};
+    atildei = {{chi / 2.0}, {1.0 / (2.0 * std::sqrt(2.0)), 1.0 / (2.0 * std::sqrt(2.0))}};
+    dtildei = {chi / 2.0, 1.0 - 1.0 / std::sqrt(2.0)};
+  }
+
+  unsigned int n_stages() const { return (unsigned int)bi.size(); }
+
+  // AdditiveRungeKuttaIntegrator.h:158-182: n_stages+1 height and 2 n_stages discharge k
+  template <typename VectorType, typename Operator>
+  void reinit(const Operator &op, VectorType &vec_ri_height, VectorType &vec_ri_discharge,
+              std::vector<VectorType> &vec_ki_height,
+              std::vector<VectorType> &vec_ki_discharge) const
+  {
+    op.initialize_vector(vec_ri_height, 0);
+    op.initialize_vector(vec_ri_discharge, 1);
+    vec_ki_height.resize(n_stages() + 1);
+    vec_ki_discharge.resize(2 * n_stages());
+    for (auto &v : vec_ki_height)
+      op.initialize_vector(v, 0);
+    for (auto &v : vec_ki_discharge)
+      op.initialize_vector(v, 1);
+  }
+
+  template <typename VectorType, typename Operator>
+  void perform_time_step(Operator &op, const double current_time, const double time_step,
+                         VectorType &solution_height, VectorType &solution_discharge,
+                         VectorType &postprocess_velocity, VectorType &vec_ri_height,
+                         VectorType &vec_ri_discharge, std::vector<VectorType> &vec_ki_height,
+                         std::vector<VectorType> &vec_ki_discharge) const
+  {
+    std::vector<double> b_i = bi;
+    for (auto &v : b_i)
+      v *= time_step;
+    std::vector<std::vector<double>> a_i = ai, at_i = atildei;
+    for (auto &row : a_i)
+      for (auto &v : row)
+        v *= time_step;
+    for (auto &row : at_i)
+      for (auto &v : row)
+        v *= time_step;
+    const VectorType empty;
+    for (unsigned int stage = 0; stage < ci.size(); ++stage)
+      {
+        const bool last = stage == ci.size() - 1;
+        op.factor_matrix = stage < dtildei.size() ? dtildei[stage] * time_step : 0.0;
+        std::vector<VectorType> vec_ri =
+          stage == 0 ? std::vector<VectorType>{solution_height, solution_discharge, empty,
+                                               postprocess_velocity}
+                     : std::vector<VectorType>{vec_ri_height, vec_ri_discharge, empty,
+                                               postprocess_velocity};
+        op.perform_stage_hydro(stage, current_time + ci[stage] * time_step,
+                               last ? &b_i[0] : &a_i[stage][0],
+                               last ? &b_i[0] : &at_i[stage][0], vec_ri, vec_ki_height,
+                               vec_ki_discharge, solution_height, solution_discharge,
+                               vec_ri_height, vec_ri_discharge);
+      }
+  }
+
+private:
+  std::vector<std::vector<double>> ai, atildei;
+  std::vector<double> bi, ci, dtildei;
 };
 
 }  // namespace oceano

@@ -1,6 +1,7 @@
-// Drop-in check of include/oceano_operator.hpp: the reference-shaped time loop
-// (ExplicitRungeKuttaIntegrator with vec_ri copies, OceanoOperator::perform_stage_hydro)
-// gives bit-identical states to direct C-ABI stage calls, and C-ABI failures surface as
+// Drop-in check of include/oceano_operator.hpp: the reference-shaped time loops
+// (ExplicitRungeKuttaIntegrator and AdditiveRungeKuttaIntegrator with vec_ri copies,
+// OceanoOperator::perform_stage_hydro and its additive overload with factor_matrix) give
+// bit-identical states to direct C-ABI stage calls, and C-ABI failures surface as
 // exceptions with the status code.
 #include <cmath>
 #include <cstdio>
@@ -126,6 +127,67 @@ int main()
   std::printf("adapter vs C ABI after 10 RK4 steps: bitwise %s, max|diff| %.3e, cells %lld\n",
               same ? "identical" : "DIFFERENT", dmax, (long long)info[0]);
 
+  // (c) the additive (IMEX) integrator through the adapter vs direct C-ABI calls, with a
+  // Manning friction field so that the stiff terms are active
+  bool same_ark = false;
+  {
+    oceano::OceanoOperator<DeviceVector> opf(desc, prm);
+    opf.initialize_data_at_quadrature([](double x, double, double *out) {
+      for (int c = 0; c < 5; ++c)
+        out[c] = 0.0;
+      out[1] = 0.05 * (1.0 + 0.5 * std::cos(3.0 * x));
+    });
+    oceano::AdditiveRungeKuttaIntegrator ark;
+    DeviceVector ah = h0, aq = q0, avel, arh, arq;
+    std::vector<DeviceVector> akh, akq;
+    ark.reinit(opf, arh, arq, akh, akq);
+    for (int n = 0; n < 5; ++n)
+      ark.perform_time_step(opf, n * dt, dt, ah, aq, avel, arh, arq, akh, akq);
+    // direct: two alternating r registers
+    const double chi = 2.0 - std::sqrt(2.0);
+    const double ea[2][2] = {{chi * dt}, {0.5 * dt, 0.5 * dt}};
+    const double ta[2][2] = {{chi / 2.0 * dt},
+                             {1.0 / (2.0 * std::sqrt(2.0)) * dt, 1.0 / (2.0 * std::sqrt(2.0)) * dt}};
+    const double eb[3] = {(0.5 - chi / 4.0) * dt, (0.5 - chi / 4.0) * dt, chi / 2.0 * dt};
+    const double fm[3] = {chi / 2.0 * dt, (1.0 - 1.0 / std::sqrt(2.0)) * dt, 0.0};
+    DeviceVector ch = h0, cq = q0, r3h[2], r3q[2];
+    std::vector<DeviceVector> k3h(4), k3q(6);
+    for (int i = 0; i < 2; ++i)
+      {
+        opf.initialize_vector(r3h[i], 0);
+        opf.initialize_vector(r3q[i], 1);
+      }
+    for (auto &v : k3h)
+      opf.initialize_vector(v, 0);
+    for (auto &v : k3q)
+      opf.initialize_vector(v, 1);
+    for (int n = 0; n < 5; ++n)
+      for (unsigned s = 0; s < 3; ++s)
+        {
+          double *khp[4], *kqp[6];
+          for (int j = 0; j < 4; ++j)
+            khp[j] = k3h[j].get_values();
+          for (int j = 0; j < 6; ++j)
+            kqp[j] = k3q[j].get_values();
+          const double *rih = s == 0 ? ch.get_values() : r3h[s % 2].get_values();
+          const double *riq = s == 0 ? cq.get_values() : r3q[s % 2].get_values();
+          oceano::check(oc_dg_perform_stage_hydro_ark(
+                          opf.handle(), s, 0.0, s == 2 ? eb : ea[s], s == 2 ? eb : ta[s], s + 1,
+                          fm[s], rih, riq, nullptr, khp, 4, kqp, 6, ch.get_values(),
+                          cq.get_values(), r3h[(s + 1) % 2].get_values(),
+                          r3q[(s + 1) % 2].get_values(), nullptr),
+                        "ark stage");
+        }
+    const auto X = host(ah), Y = host(ch), Xq = host(aq), Yq = host(cq);
+    same_ark = std::memcmp(X.data(), Y.data(), X.size() * 8) == 0 &&
+               std::memcmp(Xq.data(), Yq.data(), Xq.size() * 8) == 0;
+    double d = 0;
+    for (size_t i = 0; i < Xq.size(); ++i)
+      d = std::fmax(d, std::fabs(Xq[i] - Yq[i]));
+    std::printf("ARK adapter vs C ABI after 5 IMEX steps: bitwise %s, max|diff q| %.3e\n",
+                same_ark ? "identical" : "DIFFERENT", d);
+  }
+
   // errors become exceptions carrying the status code
   int code = 0;
   oc_mesh *m2 = nullptr;
@@ -155,7 +217,7 @@ int main()
     }
   oc_mesh_destroy(m2);
   oc_mesh_destroy(mesh);
-  const bool ok = same && code == OC_ERR_BOUNDARY;
+  const bool ok = same && same_ark && code == OC_ERR_BOUNDARY;
   std::printf("%s\n", ok ? "OK" : "FAILED");
   return ok ? 0 : 1;
 }
