@@ -135,6 +135,12 @@ def lib():
     """Load liboceano_dg.so (once) and declare every exported function."""
     global _lib
     if _lib is None:
+        # torch's HIP runtime first: loading the library before torch brings in a second
+        # runtime, and the process then sees no device
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 "oceano_amd: %s is missing; build it with `python oceano_amd/build.py` "
