@@ -165,3 +165,32 @@ def test_ark_argument_errors(cuda):
         op.perform_stage_hydro_ark(1, 0.0, [1e-3, 1e-3], [1e-3, 1e-3], 1e-3, [r_h, r_q],
                                    ki_h, ki_q, w_h, w_q, r_h, r_q)
     torch.cuda.synchronize()
+
+
+def test_ark_viscous_matches_oracle(cuda):
+    """viscosity 1e-3 on the rough hanging vortex: the stage reads the velocity of the
+    step's initial state, updated after each step as main.cc:1573-1575 does"""
+    P = _rough_vortex(2, 0.1)
+    P.viscosity = 1e-3
+    print("[ark] viscous p=2 cells=%d" % P.mesh.n_cells, flush=True)
+    op, o, h, q = make_pair(P)
+    dt = 2e-4
+    w_h, w_q = to_dev(op, h, 1), to_dev(op, q, 2)
+    vel = op.velocity(w_h, w_q)
+    rk = AdditiveRungeKuttaIntegrator()
+    rk.reinit(op)
+    oh, oq = h.copy(), q.copy()
+    S = len(ark_tableau()[1])
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(2 * S)]
+    for i in range(3):
+        rk.perform_time_step(op, i * dt, dt, w_h, w_q, vel)
+        op.velocity(w_h, w_q, vel)
+        o.time_step_ark(dt, oh, oq, ki_h, ki_q, o.zeros(1), o.zeros(2),
+                        v=o.velocity_field(oh, oq))
+    torch.cuda.synchronize()
+    eh, eq = rel_l2(to_host(op, w_h, 1), oh), rel_l2(to_host(op, w_q, 2), oq)
+    print("  3 viscous IMEX steps: zeta %.2e q %.2e" % (eh, eq))
+    assert eh < 1e-12 and eq < 1e-12
+    # the viscous term acted
+    assert rel_l2(oq, _oracle_steps(make_pair(_rough_vortex(2, 0.1))[1], h, q, dt, 3)[1]) > 1e-10

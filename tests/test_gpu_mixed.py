@@ -120,3 +120,34 @@ def test_mixed_wet_dry_takes_fp64_depth_solve(cuda):
     eh, eq = rel_l2(gh, w_h), rel_l2(gq, w_q)
     print("  dambreak mixed vs fp64: zeta %.2e q %.2e" % (eh, eq))
     assert eh < 1e-5 and eq < 1e-3
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_mixed_viscous_run_matches_fp64(cuda, degree):
+    """viscosity 1e-3 (the velocity-gradient flux in FP32, the velocity itself an FP64
+    input updated after every step): 10 RK4 steps within 1e-6 of the FP64 oracle"""
+    from oceano_amd.integrator import DeviceIntegrator
+    P = problems.vortex(degree=degree, refine=1, refine_fraction=0.3, seed=5)
+    P.viscosity = 1e-3
+    print("[mixed] viscous p=%d cells=%d" % (degree, P.mesh.n_cells), flush=True)
+    op, o, h, q = make_pair(P, precision="mixed")
+    dt = 2e-4
+    rows, _ = butcher_rows("stage_4_order_4", dt)
+    S = len(rows)
+    w_h, w_q = to_dev(op, h, 1), to_dev(op, q, 2)
+    vel = op.velocity(w_h, w_q)
+    rk = DeviceIntegrator(op, "stage_4_order_4")
+    oh, oq = h.copy(), q.copy()
+    ki_h = [o.zeros(1) for _ in range(S + 1)]
+    ki_q = [o.zeros(2) for _ in range(S + 1)]
+    for _ in range(10):
+        rk.step(dt, w_h, w_q, vel)
+        o.time_step(rows, oh, oq, ki_h, ki_q, o.zeros(1), o.zeros(2),
+                    v=o.velocity_field(oh, oq))
+    torch.cuda.synchronize()
+    gh, gq = to_host(op, w_h, 1), to_host(op, w_q, 2)
+    eh, eq, ei = rel_l2(gh, oh), rel_l2(gq, oq), rel_l2(gq - q, oq - q)
+    print("  mixed vs fp64: zeta %.2e q %.2e, q increment %.2e" % (eh, eq, ei))
+    assert eh < 1e-6 and eq < 1e-6
+    assert ei < 1e-4
+    assert rel_l2(to_host(op, vel, 2), o.velocity_field(oh, oq)) < 1e-6
