@@ -138,4 +138,31 @@ __device__ __forceinline__ double fld(const Field &f, size_t i)
   return f.ptr ? f.ptr[i] : f.cval;
 }
 
+// A field copied into VGPRs ahead of a hot loop.  Its pointer and value are wave-uniform
+// kernel arguments; the stage kernels hold more uniform values than fit in the SGPRs, and
+// the compiler then parks the argument block in VGPR lanes and restores a whole group of
+// 16 SGPRs (one v_readlane each) at every read of a field inside a loop.  Copied into
+// VGPRs once, a read is a select (or the load) and nothing more.
+template <typename R>
+struct FieldV
+{
+  const double *ptr;
+  R cval;
+  bool stored;
+};
+
+template <typename R>
+__device__ __forceinline__ FieldV<R> in_vgprs(const Field &f)
+{
+  FieldV<R> v{f.ptr, R(f.cval), f.ptr != nullptr};
+  asm volatile("" : "+v"(v.ptr), "+v"(v.cval));
+  return v;
+}
+
+template <typename R>
+__device__ __forceinline__ R fld(const FieldV<R> &f, size_t i)
+{
+  return f.stored ? R(f.ptr[i]) : f.cval;
+}
+
 }  // namespace oceano

@@ -158,6 +158,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
           const int bk = O.bkind[bf];
           const Vec2<R> n = {f == 0 ? R(-1) : f == 1 ? R(1) : R(0),
                              f == 2 ? R(-1) : f == 3 ? R(1) : R(0)};
+          const FieldV<R> bz = in_vgprs<R>(O.bdryz);
 #pragma unroll 1
           for (int a = 0; a < NQ; ++a)
             {
@@ -170,7 +171,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                   yo += T.Sq[a][k] * yo_l[k];
                 }
               zo += zcr;
-              const R zb = R(fld(O.bdryz, (size_t)bf * NQ + a));
+              const R zb = fld(bz, (size_t)bf * NQ + a);
               const double *bv = O.bvals + ((size_t)bf * NQ + a) * 3;
               R val[NC];
               bdry(bk, zo, Vec2<R>{xo, yo}, n, zb, bv, val);
@@ -189,6 +190,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
         {
           const bool is_m = (f & 1);  // canonical orientation: minus side owns faces 1, 3
           const Vec2<R> n = {f < 2 ? R(1) : R(0), f < 2 ? R(0) : R(1)};
+          const FieldV<R> fz = in_vgprs<R>(O.facez);
           // neighbour traces of both subfaces in one group of loads (the second only on
           // the coarse side): one memory latency per face
           phase_barrier();
@@ -289,8 +291,8 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                     }
                   zo += zcr;
                   zn += zc2r;
-                  const R zbm = R(fld(O.facez, ((size_t)fi * NG + a) * 2 + 0));
-                  const R zbp = R(fld(O.facez, ((size_t)fi * NG + a) * 2 + 1));
+                  const R zbm = fld(fz, ((size_t)fi * NG + a) * 2 + 0);
+                  const R zbp = fld(fz, ((size_t)fi * NG + a) * 2 + 1);
                   R val[NC];
                   inner(is_m, zo, Vec2<R>{xo, yo}, zn, Vec2<R>{xn, yn}, n, zbm, zbp, val);
                   const R JxW = T.wg[a] * len;
@@ -610,6 +612,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       Racc[1][k] = 0;
     }
   const size_t c0base = (size_t)batch * NQ * NQ * CB + lane;
+  const FieldV<R> bed = in_vgprs<R>(O.cell0[0]);
 #pragma unroll 1
   for (int b = 0; b < NQ; ++b)
     {
@@ -669,7 +672,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
           zy *= ihy;
           const int q = a + NQ * b;
           const size_t pi = c0base + (size_t)q * CB;
-          const R zb = R(fld(O.cell0[0], pi));
+          const R zb = fld(bed, pi);
           const R h = M.depth(z, zb);
           const R s = M.inv_depth(h);
           const R vx = qx * s, vy = qy * s;
@@ -867,6 +870,7 @@ __device__ __forceinline__ int depth_mass(const OpArgs &O, const DegreeTables &T
   const size_t zbase = (size_t)batch * NG * NG * CB + lane;
   int ndry = 0;
   hmin = DBL_MAX;
+  const FieldV<double> bed = in_vgprs<double>(O.cell1);
 #pragma unroll 1
   for (int b = 0; b < NG; ++b)
     {
@@ -889,7 +893,7 @@ __device__ __forceinline__ int depth_mass(const OpArgs &O, const DegreeTables &T
           for (int i = 0; i < N1; ++i)
             z += T.Sg[a][i] * zs[i];
           const int q = a + NG * b;
-          const double zb = fld(O.cell1, zbase + (size_t)q * CB);
+          const double zb = fld(bed, zbase + (size_t)q * CB);
           if (COUNT)
             {
               if (z <= -zb)
@@ -926,6 +930,7 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
     const size_t zbase = (size_t)batch * NP * CB + lane;
     double zbmax = -DBL_MAX;
     int ndry = 0;
+    const FieldV<double> bed = in_vgprs<double>(O.cell1);
 #pragma unroll 1
     for (int b = 0; b < NG; ++b)
       {
@@ -946,7 +951,7 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
 #pragma unroll
             for (int i = 0; i < N1; ++i)
               z += T.Sg[a][i] * zs[i];
-            const double zb = fld(O.cell1, zbase + (size_t)(a + NG * b) * CB);
+            const double zb = fld(bed, zbase + (size_t)(a + NG * b) * CB);
             if (z <= -zb)
               {
                 zbmax = zb > zbmax ? zb : zbmax;
@@ -1105,6 +1110,7 @@ __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTa
     dev[i] = float(Z[i] - zc);
   float hmin = FLT_MAX;
   const size_t zbase = (size_t)batch * NG * NG * CB + lane;
+  const FieldV<double> bed = in_vgprs<double>(O.cell1);
 #pragma unroll 1
   for (int b = 0; b < NG; ++b)
     {
@@ -1125,7 +1131,7 @@ __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTa
 #pragma unroll
           for (int i = 0; i < N1; ++i)
             z += Tf.Sg[a][i] * zs[i];
-          const double zb = fld(O.cell1, zbase + (size_t)(a + NG * b) * CB);
+          const double zb = fld(bed, zbase + (size_t)(a + NG * b) * CB);
           const float h = float(zc + zb) + z;
           hmin = h < hmin ? h : hmin;
         }
@@ -1410,10 +1416,22 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
   store_sweeps(S.sweeps_blk, batch, k);
 }
 
+// Waves per SIMD the register allocator must leave room for (a SIMD holds 512 registers
+// per lane for all its waves): the occupancy each inviscid stage kernel reaches, pinned so
+// that a few more live values do not silently drop it by a wave.
+template <int P, bool VISC, typename R>
+constexpr int stage_min_waves()
+{
+  return VISC ? 1 : std::is_same<R, float>::value ? (P == 1 ? 5 : P == 2 ? 4 : P == 3 ? 3 : 2)
+                                                  : (P == 1 ? 3 : P == 2 ? 2 : 1);
+}
+
 // the whole stage in one launch: work items w = (batch, kind) are dealt so that each XCD
 // holds a contiguous range of batches; kind 0 = height stage, kind 1 = discharge stage
 template <int P, bool VISC, typename R>
-__global__ __launch_bounds__(CB) void k_stage(OpArgs O, StageArgs H, StageArgs Q)
+__global__ __launch_bounds__(CB, (stage_min_waves<P, VISC, R>())) void k_stage(OpArgs O,
+                                                                               StageArgs H,
+                                                                               StageArgs Q)
 {
   __shared__ R lds[discharge_lds_words<P, VISC>()];
   const int w = logical_batch(blockIdx.x, 2 * O.n_batches, O.xcd_swizzle);
