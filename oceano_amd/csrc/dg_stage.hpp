@@ -1426,8 +1426,17 @@ constexpr int stage_min_waves()
                                                   : (P == 1 ? 3 : P == 2 ? 2 : 1);
 }
 
+// Kind of work item w (0 = height stage, 1 = discharge stage) of batch w / 2: the parity of
+// the number of set bits of w (the Thue-Morse sequence).  Within an XCD the hardware deals
+// consecutive workgroups round-robin over shader engines, then CUs, then SIMDs, all with
+// power-of-two periods; with kind = w & 1 every CU received one kind only (measured: 128
+// CUs ran only height items and idled while the other 128 ran the heavier discharge
+// items).  Bit parity is balanced along every power-of-two stride, and the two items of a
+// batch (w = 2b, 2b + 1) still have opposite kinds and sit next to each other.
+__device__ __forceinline__ int item_kind(int w) { return __popc((unsigned)w) & 1; }
+
 // the whole stage in one launch: work items w = (batch, kind) are dealt so that each XCD
-// holds a contiguous range of batches; kind 0 = height stage, kind 1 = discharge stage
+// holds a contiguous range of batches
 template <int P, bool VISC, typename R>
 __global__ __launch_bounds__(CB, (stage_min_waves<P, VISC, R>())) void k_stage(OpArgs O,
                                                                                StageArgs H,
@@ -1435,7 +1444,7 @@ __global__ __launch_bounds__(CB, (stage_min_waves<P, VISC, R>())) void k_stage(O
 {
   __shared__ R lds[discharge_lds_words<P, VISC>()];
   const int w = logical_batch(blockIdx.x, 2 * O.n_batches, O.xcd_swizzle);
-  if (w & 1)
+  if (item_kind(w))
     discharge_block<P, VISC, 0, R>(O, Q, w >> 1, lds);
   else
     height_stage_block<P, R>(O, H, w >> 1);
