@@ -23,6 +23,7 @@ against 8 TB/s; `eager_value` is that pass's rate); `cpu_baseline` times the CPU
 restatement of the same stage (single core) on a bounded sample of the same workload.
 """
 import argparse
+import gc
 import json
 
 import numpy as np
@@ -38,7 +39,7 @@ DEFAULT = "vortex_p3_hanging_97k"
 # HBM bytes per launch of the stage kernel on the default workload, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 512 MiB copy
 # (profiles/k_stage_vortex_p3_hanging_97k.md)
-MEASURED_TRAFFIC = {DEFAULT: {"fp64": 264.9e6, "mixed": 206.6e6}}
+MEASURED_TRAFFIC = {DEFAULT: {"fp64": 279.6e6, "mixed": 208.2e6}}
 
 
 def workload(name):
@@ -236,7 +237,7 @@ def main():
         # independent operators (the ensemble's degree groups) replay on streams of their
         # own, so that one group's kernels overlap another's tail
         streams = [side] + [torch.cuda.Stream() for _ in G[1:]]
-        torch.cuda.synchronize()
+        quiesce()
         t0 = time.perf_counter()
         for g, st in zip(G, streams):
             with torch.cuda.stream(st):
@@ -248,6 +249,7 @@ def main():
             st.synchronize()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        gc.enable()
         for g in G:
             g["steps"] += full
         if rest:
@@ -256,13 +258,14 @@ def main():
         # each kernel on the launch stream
         for g in G:
             g["op"].set_timing(True)
-        torch.cuda.synchronize()
+        quiesce()
         t1 = time.perf_counter()
         for k in range(n_steps_timed):
             for g in G:
                 stage(g, k)
         torch.cuda.synchronize()
         eager = time.perf_counter() - t1
+        gc.enable()
         kern = {}
         for g in G:
             op = g["op"]
@@ -380,6 +383,16 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def quiesce():
+    """Before a timed region: collect garbage now and keep the collector off until the region
+    ends (gc.enable()), so that no operator of an earlier leg is freed inside it -- freeing
+    device memory synchronises the device and stalls the launch pipeline."""
+    import torch
+    gc.collect()
+    gc.disable()
+    torch.cuda.synchronize()
+
+
 def run_ark(problem, dt, precision, n_stages):
     """The additive (IMEX) stage_3_order_2 scheme on the same workload: eager stages (five
     kernels each: L_zeta, explicit L_q, stiff source, depth solve, modified-mass update)
@@ -396,12 +409,13 @@ def run_ark(problem, dt, precision, n_stages):
     steps = max(1, n_stages // S)
     for i in range(2):
         rk.perform_time_step(op, i * dt, dt, w_h, w_q)
-    torch.cuda.synchronize()
+    quiesce()
     t0 = time.perf_counter()
     for i in range(steps):
         rk.perform_time_step(op, i * dt, dt, w_h, w_q)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    gc.enable()
     op.set_timing(True)
     rk.perform_time_step(op, 0.0, dt, w_h, w_q)
     torch.cuda.synchronize()
