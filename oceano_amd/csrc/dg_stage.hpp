@@ -105,10 +105,120 @@ struct LdsVec
   __device__ __forceinline__ R operator[](int k) const { return p[k * CB]; }
 };
 
+// position of the j-th (0-based) set bit of m; m has more than j set bits
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int j)
+{
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1)
+    {
+      const int c = __popcll(m & ((1ull << w) - 1));
+      if (j >= c)
+        {
+          j -= c;
+          pos += w;
+          m >>= w;
+        }
+    }
+  return pos;
+}
+
+// nodal values of a trace (N1 values per field) restricted to half `half` of the face:
+// u_j <- sum_k Psub[half][j][k] u_k
+template <int N1, typename R>
+__device__ __forceinline__ void restrict_half(const Basis<R> &T, int half, R *z, R *x, R *y)
+{
+  R zt[N1], xt[N1], yt[N1];
+#pragma unroll
+  for (int j = 0; j < N1; ++j)
+    {
+      R sz = 0, sx = 0, sy = 0;
+#pragma unroll
+      for (int k = 0; k < N1; ++k)
+        {
+          sz += T.Psub[half][j][k] * z[k];
+          sx += T.Psub[half][j][k] * x[k];
+          sy += T.Psub[half][j][k] * y[k];
+        }
+      zt[j] = sz;
+      xt[j] = sx;
+      yt[j] = sy;
+    }
+#pragma unroll
+  for (int j = 0; j < N1; ++j)
+    {
+      z[j] = zt[j];
+      x[j] = xt[j];
+      y[j] = yt[j];
+    }
+}
+
+// integral over one (sub)face of NG Gauss-Lobatto points: both traces as nodal values on
+// the (sub)face, deviations from zcr / zc2r; acc[cc][k] = sum_a Sg[a][k] inner(..)_cc JxW_a
+template <int P, int NC, typename R, typename InnerFn>
+__device__ __forceinline__ void subface_integral(const Basis<R> &T, const FieldV<R> &fz, int fi,
+                                                 R len, bool is_m, Vec2<R> n, const R *zo_s,
+                                                 const R *xo_s, const R *yo_s, const R *zn_s,
+                                                 const R *xn_s, const R *yn_s, R zcr, R zc2r,
+                                                 InnerFn &inner, R (&acc)[NC][P + 1])
+{
+  constexpr int N1 = P + 1, NG = P + 2;
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+    for (int k = 0; k < N1; ++k)
+      acc[cc][k] = 0;
+#pragma unroll 1
+  for (int a = 0; a < NG; ++a)
+    {
+      R zo = 0, xo = 0, yo = 0, zn = 0, xn = 0, yn = 0;
+#pragma unroll
+      for (int k = 0; k < N1; ++k)
+        {
+          const R m = T.Sg[a][k];
+          zo += m * zo_s[k];
+          xo += m * xo_s[k];
+          yo += m * yo_s[k];
+          zn += m * zn_s[k];
+          xn += m * xn_s[k];
+          yn += m * yn_s[k];
+        }
+      zo += zcr;
+      zn += zc2r;
+      const R zbm = fld(fz, ((size_t)fi * NG + a) * 2 + 0);
+      const R zbp = fld(fz, ((size_t)fi * NG + a) * 2 + 1);
+      R val[NC];
+      inner(is_m, zo, Vec2<R>{xo, yo}, zn, Vec2<R>{xn, yn}, n, zbm, zbp, val);
+      const R JxW = T.wg[a] * len;
+#pragma unroll
+      for (int cc = 0; cc < NC; ++cc)
+        {
+          const R v = val[cc] * JxW;
+#pragma unroll
+          for (int k = 0; k < N1; ++k)
+            acc[cc][k] += T.Sg[a][k] * v;
+        }
+    }
+}
+
 // Face terms of one cell: for every face point, `inner` / `bdry` return NC values that are
 // multiplied by JxW and integrated against this cell's face basis into Racc[NC][nd].
 // Own data: Z (free-surface deviation from zc), QX, QY; neighbour traces are gathered
 // from the FP64 vectors and centred on the neighbour's first nodal value.
+//
+// The lanes of a wave are 64 cells.  A coarse face (two fine neighbours) has a second
+// subface that only its own cell integrates; in the per-face loop a wave in which any cell
+// has such a face runs that second subface for all 64 lanes (on a mesh with 11 % of its
+// faces hanging and cells in the caller's order, every wave does, on every face).  With
+// COMPACT (the FP32 path) the second subfaces of the whole wave are instead listed after
+// the face loop, face-major, and dealt one per lane: a lane integrates another cell's
+// second subface from the FP64 vectors (the same values that cell holds) and hands the
+// result back by a cross-lane read.  Each cell adds its four faces' second-subface
+// contributions in face order, so the result does not depend on which lane computed them.
+// The active lanes of a wave are a prefix (cells batch*64 + lane < n_cells), which is what
+// lets lane i take item i.  The FP64 path (one wave per SIMD, cells sorted by face
+// signature so that waves rarely diverge) cannot hide the compacted pass's extra load
+// latency and keeps the second subface in the face loop.
 template <int P, int NC, typename R, typename ZA, typename QA, typename InnerFn, typename BdryFn>
 __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, int batch,
                                            int lane, R hx, R hy, const ZA &Z, const QA &QX,
@@ -119,23 +229,29 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                                            BdryFn bdry)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P), NG = P + 2;
+  constexpr bool COMPACT = std::is_same<R, float>::value;
+  constexpr int NS = COMPACT ? 1 : 2;  // subfaces integrated in the face loop
   const R zcr = R(zc);
-  // the face tables of all four faces in one group of loads: each subface then waits for
-  // one memory latency (its neighbour's traces) instead of three dependent ones
-  int infos[4], nbrs[4][2], fids[4][2];
+  // the face tables of all four faces in one group of loads: each face then waits for one
+  // memory latency (its neighbour's traces) instead of three dependent ones
+  int infos[4], nbrs[4][NS], fids[4][NS];
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     {
       const size_t fo = ((size_t)batch * 4 + f);
       infos[f] = O.finfo[fo * CB + lane];
 #pragma unroll
-      for (int s = 0; s < 2; ++s)
+      for (int s = 0; s < NS; ++s)
         {
           nbrs[f][s] = O.nbr[(fo * 2 + s) * CB + lane];
           fids[f][s] = O.fid[(fo * 2 + s) * CB + lane];
         }
     }
   issue_barrier();
+  unsigned long long coarse[4];  // lanes whose face f is the coarse side of a hanging face
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+    coarse[f] = COMPACT ? __ballot((infos[f] & 3) == FK_COARSE) : 0ull;
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     {
@@ -191,12 +307,12 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
           const bool is_m = (f & 1);  // canonical orientation: minus side owns faces 1, 3
           const Vec2<R> n = {f < 2 ? R(1) : R(0), f < 2 ? R(0) : R(1)};
           const FieldV<R> fz = in_vgprs<R>(O.facez);
-          // neighbour traces of both subfaces in one group of loads (the second only on
-          // the coarse side): one memory latency per face
+          // neighbour traces of the subfaces in one group of loads (the second only on the
+          // coarse side, and only where it is integrated here): one memory latency per face
           phase_barrier();
-          double zg[2][N1], xg[2][N1], yg[2][N1], zc2[2];
+          double zg[NS][N1], xg[NS][N1], yg[NS][N1], zc2[NS];
 #pragma unroll
-          for (int s = 0; s < 2; ++s)
+          for (int s = 0; s < NS; ++s)
             if (s == 0 || kind == FK_COARSE)
               {
                 const int b2 = nbrs[f][s] / CB, l2 = nbrs[f][s] % CB;
@@ -212,15 +328,14 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
               }
           issue_barrier();
 #pragma unroll
-          for (int s = 0; s < 2; ++s)
+          for (int s = 0; s < NS; ++s)
             {
               if (s == 1 && kind != FK_COARSE)
                 break;
               phase_barrier();
-              const int fi = fids[f][s];
               // both traces as nodal values on this (sub)face: the coarse side restricts
               // its own trace to half s, the fine side the neighbour's to half nsub, so
-              // that every face point below uses the one (wave-uniform) table Sg
+              // that every face point uses the one (wave-uniform) table Sg
               R zo_s[N1], xo_s[N1], yo_s[N1], zn_s[N1], xn_s[N1], yn_s[N1];
 #pragma unroll
               for (int k = 0; k < N1; ++k)
@@ -232,79 +347,19 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                   xn_s[k] = R(xg[s][k]);
                   yn_s[k] = R(yg[s][k]);
                 }
-              auto restrict3 = [&](int half, R *z3, R *x3, R *y3) {
-                R zt[N1], xt[N1], yt[N1];
-#pragma unroll
-                for (int j = 0; j < N1; ++j)
-                  {
-                    R sz = 0, sx = 0, sy = 0;
-#pragma unroll
-                    for (int k = 0; k < N1; ++k)
-                      {
-                        sz += T.Psub[half][j][k] * z3[k];
-                        sx += T.Psub[half][j][k] * x3[k];
-                        sy += T.Psub[half][j][k] * y3[k];
-                      }
-                    zt[j] = sz;
-                    xt[j] = sx;
-                    yt[j] = sy;
-                  }
-#pragma unroll
-                for (int j = 0; j < N1; ++j)
-                  {
-                    z3[j] = zt[j];
-                    x3[j] = xt[j];
-                    y3[j] = yt[j];
-                  }
-              };
               if (kind == FK_COARSE)
-                restrict3(s, zo_s, xo_s, yo_s);
+                restrict_half<N1>(T, s, zo_s, xo_s, yo_s);
               else if (kind == FK_FINE)
                 {
                   if (nsub)
-                    restrict3(1, zn_s, xn_s, yn_s);
+                    restrict_half<N1>(T, 1, zn_s, xn_s, yn_s);
                   else
-                    restrict3(0, zn_s, xn_s, yn_s);
+                    restrict_half<N1>(T, 0, zn_s, xn_s, yn_s);
                 }
-              const R zc2r = R(zc2[s]);
               const R len = (kind == FK_COARSE) ? R(0.5) * flen : flen;
               R acc[NC][N1];
-#pragma unroll
-              for (int cc = 0; cc < NC; ++cc)
-#pragma unroll
-                for (int k = 0; k < N1; ++k)
-                  acc[cc][k] = 0;
-#pragma unroll 1
-              for (int a = 0; a < NG; ++a)
-                {
-                  R zo = 0, xo = 0, yo = 0, zn = 0, xn = 0, yn = 0;
-#pragma unroll
-                  for (int k = 0; k < N1; ++k)
-                    {
-                      const R m = T.Sg[a][k];
-                      zo += m * zo_s[k];
-                      xo += m * xo_s[k];
-                      yo += m * yo_s[k];
-                      zn += m * zn_s[k];
-                      xn += m * xn_s[k];
-                      yn += m * yn_s[k];
-                    }
-                  zo += zcr;
-                  zn += zc2r;
-                  const R zbm = fld(fz, ((size_t)fi * NG + a) * 2 + 0);
-                  const R zbp = fld(fz, ((size_t)fi * NG + a) * 2 + 1);
-                  R val[NC];
-                  inner(is_m, zo, Vec2<R>{xo, yo}, zn, Vec2<R>{xn, yn}, n, zbm, zbp, val);
-                  const R JxW = T.wg[a] * len;
-#pragma unroll
-                  for (int cc = 0; cc < NC; ++cc)
-                    {
-                      const R v = val[cc] * JxW;
-#pragma unroll
-                      for (int k = 0; k < N1; ++k)
-                        acc[cc][k] += T.Sg[a][k] * v;
-                    }
-                }
+              subface_integral<P, NC, R>(T, fz, fids[f][s], len, is_m, n, zo_s, xo_s, yo_s,
+                                         zn_s, xn_s, yn_s, zcr, R(zc2[s]), inner, acc);
               // test functions of the cell's face: the transposed restriction on the
               // coarse side
               if (kind == FK_COARSE)
@@ -331,6 +386,99 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                 }
             }
         }
+    }
+  // second subfaces of the coarse faces, compacted over the wave (face-major item order)
+  const int c0 = __popcll(coarse[0]), c1 = c0 + __popcll(coarse[1]);
+  const int c2 = c1 + __popcll(coarse[2]), total = c2 + __popcll(coarse[3]);
+  if (!COMPACT || total == 0)
+    return;
+  const int n_act = __popcll(__ballot(1));
+  const int first[4] = {0, c0, c1, c2};
+  for (int base = 0; base < total; base += n_act)
+    {
+      phase_barrier();
+      R res[NC][N1];
+#pragma unroll
+      for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+          res[cc][k] = 0;
+      const int item = base + lane;
+      if (item < total)
+        {
+          const int f = item < c0 ? 0 : item < c1 ? 1 : item < c2 ? 2 : 3;
+          const unsigned long long m =
+            f == 0 ? coarse[0] : f == 1 ? coarse[1] : f == 2 ? coarse[2] : coarse[3];
+          const int owner = nth_set_bit(m, item - (f == 0 ? 0 : f == 1 ? c0 : f == 2 ? c1 : c2));
+          const size_t fo = (size_t)batch * 4 + f;
+          const int nb = O.nbr[(fo * 2 + 1) * CB + owner];
+          const int fi = O.fid[(fo * 2 + 1) * CB + owner];
+          const double hsz = O.hxy[((size_t)batch * 2 + (f < 2 ? 1 : 0)) * CB + owner];
+          const int b2 = nb / CB, l2 = nb % CB;
+          double zt[N1], xt[N1], yt[N1], zg[N1], xg[N1], yg[N1];
+          const double zco = centred<R>() ? src_h[vix(batch, 1, 0, ND, 0, owner)] : 0.0;
+          const double zc2 = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
+#pragma unroll
+          for (int k = 0; k < N1; ++k)
+            {
+              const int d = face_dof<N1>(f, k), e = face_dof<N1>(f ^ 1, k);
+              zt[k] = src_h[vix(batch, 1, 0, ND, d, owner)];
+              xt[k] = src_q[vix(batch, 2, 0, ND, d, owner)];
+              yt[k] = src_q[vix(batch, 2, 1, ND, d, owner)];
+              zg[k] = src_h[vix(b2, 1, 0, ND, e, l2)];
+              xg[k] = src_q[vix(b2, 2, 0, ND, e, l2)];
+              yg[k] = src_q[vix(b2, 2, 1, ND, e, l2)];
+            }
+          issue_barrier();
+          R zo_s[N1], xo_s[N1], yo_s[N1], zn_s[N1], xn_s[N1], yn_s[N1];
+#pragma unroll
+          for (int k = 0; k < N1; ++k)
+            {
+              zo_s[k] = R(zt[k] - zco);
+              xo_s[k] = R(xt[k]);
+              yo_s[k] = R(yt[k]);
+              zn_s[k] = R(zg[k] - zc2);
+              xn_s[k] = R(xg[k]);
+              yn_s[k] = R(yg[k]);
+            }
+          restrict_half<N1>(T, 1, zo_s, xo_s, yo_s);
+          const Vec2<R> n = {f < 2 ? R(1) : R(0), f < 2 ? R(0) : R(1)};
+          const FieldV<R> fz = in_vgprs<R>(O.facez);
+          R acc[NC][N1];
+          subface_integral<P, NC, R>(T, fz, fi, R(0.5) * R(hsz), (f & 1) != 0, n, zo_s, xo_s,
+                                     yo_s, zn_s, xn_s, yn_s, R(zco), R(zc2), inner, acc);
+#pragma unroll
+          for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+            for (int k = 0; k < N1; ++k)
+              {
+                R t = 0;
+#pragma unroll
+                for (int j = 0; j < N1; ++j)
+                  t += T.Psub[1][j][k] * acc[cc][j];
+                res[cc][k] = t;
+              }
+        }
+      // each cell takes the results of its own items computed in this pass, face by face
+#pragma unroll
+      for (int f = 0; f < 4; ++f)
+        if (coarse[f])
+          {
+            const bool mine = (coarse[f] >> lane) & 1;
+            const int helper =
+              first[f] + __popcll(coarse[f] & ((1ull << lane) - 1)) - base;
+            const bool here = mine && helper >= 0 && helper < n_act;
+            const int src = here ? helper : lane;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+              for (int k = 0; k < N1; ++k)
+                {
+                  const R v = __shfl(res[cc][k], src);
+                  if (here)
+                    Racc[cc][face_dof<N1>(f, k)] += v;
+                }
+          }
     }
 }
 
@@ -451,10 +599,12 @@ __device__ __forceinline__ void height_operator(const OpArgs &O, const Basis<R> 
   face_terms<P, 1, R>(
     O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
     [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
-      if (is_m)
-        val[0] = -M.mass_flux(zo, zn, qo, qn, n, zbm, zbp);
-      else
-        val[0] = M.mass_flux(zn, zo, qn, qo, n, zbm, zbp);
+      // minus side: -F(own, neighbour); plus side: F(neighbour, own).  Selects instead of
+      // branches: the face side is a constant in the per-face loop and a lane value in the
+      // compacted second-subface pass
+      const R f = M.mass_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn, is_m ? qn : qo,
+                              n, zbm, zbp);
+      val[0] = is_m ? -f : f;
     },
     [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
       R zp;
@@ -745,20 +895,13 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   face_terms<P, 2, R>(
     O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
     [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
-      if (is_m)
-        {
-          const Vec2<R> F = M.adv_flux(zo, zn, qo, qn, n, zbm, zbp);
-          const Vec2<R> Pm = M.pres_flux(zo, zn, n, zbm, zbp);
-          val[0] = -F.x - Pm.x;
-          val[1] = -F.y - Pm.y;
-        }
-      else
-        {
-          const Vec2<R> F = M.adv_flux(zn, zo, qn, qo, n, zbm, zbp);
-          const Vec2<R> Pp = M.pres_flux(zo, zn, n, zbp, zbm);
-          val[0] = F.x + Pp.x;
-          val[1] = F.y + Pp.y;
-        }
+      // minus side: -(F(own, nbr) + P(own, nbr; zbm, zbp)); plus side: F(nbr, own) +
+      // P(own, nbr; zbp, zbm), written with selects (see the height flux)
+      const Vec2<R> F = M.adv_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn,
+                                   is_m ? qn : qo, n, zbm, zbp);
+      const Vec2<R> Pr = M.pres_flux(zo, zn, n, is_m ? zbm : zbp, is_m ? zbp : zbm);
+      val[0] = is_m ? -F.x - Pr.x : F.x + Pr.x;
+      val[1] = is_m ? -F.y - Pr.y : F.y + Pr.y;
     },
     [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
       R zp;
