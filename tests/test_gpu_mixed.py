@@ -100,11 +100,16 @@ def test_mixed_vortex_run_matches_fp64(cuda, degree):
         assert abs(e_mixed[k] - e_fp64[i]) <= 1e-3 * e_fp64[i] + 1e-8
 
 
-def test_mixed_wet_dry_takes_fp64_depth_solve(cuda):
-    """Dam break onto a dry bed: cells near the floor take the FP64 sweep; the run stays
+# p = 2, 3: the degrees whose FP64 dam break stays finite in the reference arithmetic
+# (the wet/dry parity cases)
+@pytest.mark.parametrize("degree", [2, 3])
+def test_mixed_wet_dry_takes_fp64_depth_solve(cuda, degree):
+    """Dam break onto a dry bed: cells near the floor take the FP64 sweep (inside the stage
+    kernel at p = 2, in the deferred kernel over the flagged cells at p >= 3); the run stays
     finite and close to the FP64 oracle."""
     import test_gpu_parity as T
-    P = T._dambreak_problem(2)
+    P = T._dambreak_problem(degree)
+    print("[mixed] dambreak p=%d" % degree, flush=True)
     op, o, h, q = make_pair(P, precision="mixed")
     rows, _ = butcher_rows("stage_4_order_4", 2e-3)
     mh, mq = _run(op, h, q, rows, 3)
@@ -119,7 +124,9 @@ def test_mixed_wet_dry_takes_fp64_depth_solve(cuda):
     assert np.isfinite(gh).all() and np.isfinite(gq).all()
     eh, eq = rel_l2(gh, w_h), rel_l2(gq, w_q)
     print("  dambreak mixed vs fp64: zeta %.2e q %.2e" % (eh, eq))
-    assert eh < 1e-5 and eq < 1e-3
+    # measured 6e-9 / 1e-7 at p = 2 and 3 (the dry half sits on the depth floor in FP64 on
+    # both paths; the FP32 integrals of the wet half set the difference)
+    assert eh < 1e-7 and eq < 1e-5
 
 
 @pytest.mark.parametrize("degree", [1, 3])
