@@ -39,7 +39,7 @@ DEFAULT = "vortex_p3_hanging_97k"
 # HBM bytes per launch of the stage kernel on the default workload, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 512 MiB copy
 # (profiles/k_stage_vortex_p3_hanging_97k.md)
-MEASURED_TRAFFIC = {DEFAULT: {"fp64": 278.4e6, "mixed": 231.3e6}}
+MEASURED_TRAFFIC = {DEFAULT: {"fp64": 277.5e6, "mixed": 240.2e6}}
 
 
 def workload(name):

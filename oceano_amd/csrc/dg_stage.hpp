@@ -1234,9 +1234,11 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
 // O.defer[batch] for k_depth_deferred (keeps the FP64 sweep's registers out of the fused
 // FP32 stage kernel).  Called by the active lanes; lane 0 of a batch is always active.
 // The FP32 stage leaves the FP64 depth solve of cells near the floor to k_depth_deferred
-// at p >= 3, where its registers would cut the stage kernel's occupancy; at p <= 2 the
-// stage kernel does it itself (same occupancy, one launch less per stage).
-__host__ __device__ constexpr bool defer_depth_solve(int p) { return p >= 3; }
+// at p = 4, where its registers would spill heavily (256 VGPRs, 268 B per lane); at p <= 3
+// the stage kernel does it itself.  At p = 3 that costs 47 spill slots but keeps 3 waves
+// per SIMD, and saves the second launch: measured 2 % faster per stage on the 97k-cell
+// hanging vortex.
+__host__ __device__ constexpr bool defer_depth_solve(int p) { return p >= 4; }
 
 template <int P, bool DEFER>
 __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTables &T,
