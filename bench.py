@@ -20,7 +20,10 @@ Extra fields: `roofline` for the kernel that takes most of the stage time (algor
 HBM bytes per launch / mean launch time from HIP events recorded on the launch stream
 around each kernel in a second pass over the same number of stages launched eagerly,
 against 8 TB/s; `eager_value` is that pass's rate); `cpu_baseline` times the CPU
-restatement of the same stage (single core) on a bounded sample of the same workload.
+restatement of the same stage (its loops on the OpenMP threads of the host's CPU share) on
+a bounded sample of the same workload.  `--ark` adds the additive (IMEX) stage's rate on
+the same mesh; `--steady` adds the whole steady-state run (the channel's criterion: one
+step changes the state by less than 1e-10 relative L2) per precision.
 """
 import argparse
 import gc
@@ -151,6 +154,9 @@ def main():
                     help="library option slot_chunk (cell grouping by face signature)")
     ap.add_argument("--ark", action="store_true",
                     help="also time the additive (IMEX) stage on the same workload")
+    ap.add_argument("--steady", action="store_true",
+                    help="also run the workload to steady state (the channel's criterion: one "
+                         "step changes the state by < 1e-10 relative L2) and time the whole run")
     ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
                     help="time the work-layout options per operator and keep the fastest "
                          "(auto: on for the ensemble)")
@@ -376,6 +382,8 @@ def main():
             "rel_l2_hv": float(np.linalg.norm(comp(ms, 1) - comp(fs, 1)) / qn)}
     if args.ark and not multi:
         line["ark"] = {p: run_ark(groups[0], dts[0], p, n_steps_timed) for p in precisions}
+    if args.steady and not multi:
+        line["steady"] = {p: run_steady(groups[0], dts[0], p) for p in precisions}
     if not args.no_cpu and not multi:
         line["cpu_baseline"] = cpu_baseline(groups[0], args.cpu_stages)
     else:
@@ -391,6 +399,37 @@ def quiesce():
     gc.collect()
     gc.disable()
     torch.cuda.synchronize()
+
+
+def run_steady(problem, dt, precision, tol=1e-10, chunk=2000, max_steps=400000):
+    """The whole steady-state job (C3's criterion): graph-replayed RK4 steps from the
+    initial state until one step changes (zeta, q) by less than tol in relative L2, checked
+    every `chunk` steps; stops early when the change has not halved over 10 checks (a
+    rounding floor).  Wall time includes the checks."""
+    import torch
+    from oceano_amd import problems
+    from oceano_amd.integrator import DeviceIntegrator
+    op, data, w_h, w_q = problems.setup_operator(problem, precision=precision)
+    if isinstance(dt, tuple):
+        dt = dt[1] / float("%.3g" % op.transport_speed(w_h, w_q))
+    rk = DeviceIntegrator(op)
+    history = []
+    steps, change = 0, float("inf")
+    quiesce()
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        n, change = rk.run_to_steady(w_h, w_q, dt, tol=tol, max_steps=chunk, chunk=chunk)
+        steps += n
+        history.append(change)
+        if not change == change or change < tol:
+            break
+        if len(history) > 10 and change > 0.5 * history[-11]:
+            break
+    el = time.perf_counter() - t0
+    gc.enable()
+    return {"converged": bool(change < tol), "steps": steps, "dt": dt,
+            "t_simulated": steps * dt, "last_change": change, "wall_s": el,
+            "dof_stages_per_s": problem.dofs() * 4 * steps / el}
 
 
 def run_ark(problem, dt, precision, n_stages):
