@@ -206,7 +206,9 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
  * The last stage updates w_h, w_q in place and writes r_q = M^-1 (b . kex + b . kim).
  * Boundary data come from the stores set with oc_dg_set_boundary_values, so `time` is not
  * read.  ri_vel is read only when the viscosity is nonzero.  OC_PRECISION_MIXED: ki_h[1..]
- * hold FP32 values in their first half as in the explicit stage; ki_q stay FP64. */
+ * and ki_q[1..] hold FP32 values in their first half as in the explicit stage (the explicit
+ * and stiff momentum terms are FP32 integrals, so the storage is exact); the update that
+ * combines them is FP64. */
 int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
                                   const double *factor, const double *factor_tilde,
                                   uint32_t n_factor, double factor_matrix, const double *ri_h,

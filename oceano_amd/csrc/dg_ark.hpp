@@ -12,9 +12,10 @@
 //                  (b_j kex_j + b_{j-1} kim_j), r_q = M^-1 front, w_q += r_q (2040-2071).
 // The height part of the stage and the explicit momentum integrals reuse the ERK kernels
 // (k_height_rhs, k_discharge with the friction switched off, k_height_jacobi).  FP64 for
-// the update; the integrals follow the operator's precision.  The height k vectors are
-// stored in the integral precision (as in the explicit stage), the discharge k vectors in
-// FP64 (the stiff terms and the modified-mass update are FP64).
+// the update; the integrals follow the operator's precision.  Every k vector of the stage
+// (height, explicit and stiff momentum terms) is stored in the integral precision R: on
+// the FP32 path their values are FP32 numbers, so the storage is exact and halves their
+// traffic; the modified-mass update reads them into FP64.
 #pragma once
 
 #include "dg_stage.hpp"
@@ -36,10 +37,11 @@ struct ArkArgs
 template <int P, typename R>
 __global__ __launch_bounds__(CB) void k_stiff(OpArgs O, const double *__restrict__ ri_h,
                                               const double *__restrict__ ri_q,
-                                              double *__restrict__ dst)
+                                              double *__restrict__ dst_buf)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
   const Basis<R> &T = basis<R>(P);
+  R *const dst = reinterpret_cast<R *>(dst_buf);  // the stiff k vector, stored in R
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
   const int lane = threadIdx.x;
   if (batch * CB + lane >= O.n_cells)
@@ -118,8 +120,8 @@ __global__ __launch_bounds__(CB) void k_stiff(OpArgs O, const double *__restrict
 #pragma unroll
   for (int k = 0; k < ND; ++k)
     {
-      dst[vix(batch, 2, 0, ND, k, lane)] = double(Rx[k]);
-      dst[vix(batch, 2, 1, ND, k, lane)] = double(Ry[k]);
+      dst[vix(batch, 2, 0, ND, k, lane)] = Rx[k];
+      dst[vix(batch, 2, 1, ND, k, lane)] = Ry[k];
     }
 }
 
@@ -185,9 +187,12 @@ __device__ __forceinline__ void tensor_apply(const double (&A)[MAXN1][MAXN1], co
       }
 }
 
-template <int P>
+// KT: storage type of the discharge k vectors (the integral precision)
+template <int P, typename KT>
 __global__ __launch_bounds__(CB) void k_ark_update(OpArgs O, ArkArgs A)
 {
+  auto kex = [&](int j, size_t i) { return double(reinterpret_cast<const KT *>(A.kex[j])[i]); };
+  auto kim = [&](int j, size_t i) { return double(reinterpret_cast<const KT *>(A.kim[j])[i]); };
   constexpr int N1 = P + 1, ND = N1 * N1;
   const DegreeTables &T = basis<double>(P);
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
@@ -250,27 +255,27 @@ __global__ __launch_bounds__(CB) void k_ark_update(OpArgs O, ArkArgs A)
         {
 #pragma unroll
           for (int k = 0; k < ND; ++k)
-            F[k] = A.factor[0] * A.kex[0][base + (size_t)k * CB];
+            F[k] = A.factor[0] * kex(0, base + (size_t)k * CB);
           for (int j = 1; j <= A.stage; ++j)
             {
 #pragma unroll
               for (int k = 0; k < ND; ++k)
-                F[k] += A.factor[j] * A.kex[j][base + (size_t)k * CB] +
-                        A.factor[j - 1] * A.kim[j - 1][base + (size_t)k * CB];
+                F[k] += A.factor[j] * kex(j, base + (size_t)k * CB) +
+                        A.factor[j - 1] * kim(j - 1, base + (size_t)k * CB);
             }
         }
       else
         {
 #pragma unroll
           for (int k = 0; k < ND; ++k)
-            F[k] = A.factor[0] * A.kex[0][base + (size_t)k * CB] +
-                   A.factor_tilde[0] * A.kim[0][base + (size_t)k * CB];
+            F[k] = A.factor[0] * kex(0, base + (size_t)k * CB) +
+                   A.factor_tilde[0] * kim(0, base + (size_t)k * CB);
           for (int j = 1; j <= A.stage; ++j)
             {
 #pragma unroll
               for (int k = 0; k < ND; ++k)
-                F[k] += A.factor[j] * A.kex[j][base + (size_t)k * CB] +
-                        A.factor_tilde[j] * A.kim[j][base + (size_t)k * CB];
+                F[k] += A.factor[j] * kex(j, base + (size_t)k * CB) +
+                        A.factor_tilde[j] * kim(j, base + (size_t)k * CB);
             }
           // + M w_q at Gauss(p+1): S^T W S w
           double U[ND], V[ND], MW[ND];
@@ -341,9 +346,9 @@ int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stag
     Oe.has_friction = 0;
   timed_begin(op, s, &ev);
   if (O.nu != 0.0)
-    hipLaunchKernelGGL((k_discharge<P, true, 2, R>), grid, block, 0, s, Oe, D);
+    hipLaunchKernelGGL((k_discharge<P, true, 4, R>), grid, block, 0, s, Oe, D);
   else
-    hipLaunchKernelGGL((k_discharge<P, false, 2, R>), grid, block, 0, s, Oe, D);
+    hipLaunchKernelGGL((k_discharge<P, false, 4, R>), grid, block, 0, s, Oe, D);
   if (!last)
     hipLaunchKernelGGL((k_stiff<P, R>), grid, block, 0, s, O, ri_h, ri_q, ki_q[2 * stage + 2]);
   timed_end(op, s, TK_DISCHARGE, ev);
@@ -379,7 +384,7 @@ int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stag
   A.rdst = r_q;
   A.wq_out = w_q;
   timed_begin(op, s, &ev);
-  hipLaunchKernelGGL((k_ark_update<P>), grid, block, 0, s, O, A);
+  hipLaunchKernelGGL((k_ark_update<P, R>), grid, block, 0, s, O, A);
   timed_end(op, s, TK_RK_UPDATE, ev);
   HIP_TRY(hipGetLastError());
   return OC_OK;

@@ -677,8 +677,9 @@ __global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__res
 
 // ---------------------------------------------------------------------------------------
 // L_q of one batch; MODE 0: stage (M^-1, RK update), 1: apply (M^-1 -> dst), 2: rhs,
-// 3: stage of the split path (M^-1 -> k vector).  The batch's own zeta, q (and velocity)
-// are staged in LDS, lane-interleaved.
+// 3: stage of the split path (M^-1 -> k vector), 4: rhs into a k vector (the explicit
+// momentum integral of the additive stage).  The batch's own zeta, q (and velocity) are
+// staged in LDS, lane-interleaved.
 //
 // k vectors of a stage (ki_h[1..], ki_q[1..]) are stored in the integral precision R: the
 // FP32 path's k values are FP32 numbers, so FP32 storage is exact and halves their
@@ -696,7 +697,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
   // dst of a stage is a k vector (stored in R); of apply / rhs a caller vector (FP64)
-  using DT = typename std::conditional<MODE == 0 || MODE == 3, R, double>::type;
+  using DT = typename std::conditional<MODE == 0 || MODE == 3 || MODE == 4, R, double>::type;
   DT *const dst = reinterpret_cast<DT *>(S.dst);
   const Basis<R> &T = basis<R>(P);
   const int lane = threadIdx.x;
@@ -916,7 +917,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       val[0] = -F.x;
       val[1] = -F.y;
     });
-  if (MODE == 2)
+  if (MODE == 2 || MODE == 4)
     {
 #pragma unroll
       for (int c = 0; c < 2; ++c)
