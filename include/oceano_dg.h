@@ -150,12 +150,30 @@ int oc_dg_n_points(const oc_dg *op, int32_t store, int64_t *n_points);
 int oc_dg_quadrature_points(const oc_dg *op, int32_t store, double *xy);
 int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values);
 
-/* Boundary data at the boundary quadrature points, [n_bdry][nq][3]: the values of the
- * boundary-condition Function at its current time (OceanDG.h:1146-1184 evaluate them per
- * quadrature point; here the host tabulates them, per stage when they depend on time).
- * Component 0 = free surface, 1..2 = velocity (supercritical inflow, absorbing outflow);
- * component 1 = the discharge magnitude for OC_BC_DISCHARGE_INFLOW.  Host memory. */
+/* Boundary data at the boundary quadrature points (OC_STORE_BOUNDARY points),
+ * [n_bdry][nq][3] per time: the values of the boundary-condition Functions, which the
+ * reference evaluates per quadrature point (OceanDG.h:1146-1184) after setting their time to
+ * the stage time (OceanDG.h:1798-1803; apply: 1700-1705).  Component 0 = free surface,
+ * 1..2 = velocity (supercritical inflow, absorbing outflow); component 1 = the discharge
+ * magnitude for OC_BC_DISCHARGE_INFLOW.  Host memory; both calls synchronise the device
+ * (a stage still running may read the previous data).
+ *   oc_dg_set_boundary_values  time-independent data: one entry used at every time.
+ *   oc_dg_set_boundary_table   values at n_times times (finite, non-decreasing; repeated
+ *                              times allowed), [n_times][n_bdry][nq][3].  Every call that
+ *                              takes a `time` (stage, additive stage, apply, rhs) uses the
+ *                              entry whose time equals it to within 1e-12 max(1, |t|), the
+ *                              first such entry, and fails with OC_ERR_ARGUMENT when there is
+ *                              none.  A time loop replayed from a graph (oc_rk_replay) reads
+ *                              entries e0 + r S + s for step r and stage s, e0 the entry at
+ *                              the loop's current time: tabulate the stage times
+ *                              t + r dt + c_s dt in that order.
+ * Operators whose boundary faces are all walls or supercritical outflows read no data, and
+ * their stages accept any time.  The reference sets the time of the inflow Functions only
+ * (supercritical, height, discharge inflow); absorbing-outflow data stay at the time they
+ * had, so a caller that reproduces it tabulates them at that fixed time. */
 int oc_dg_set_boundary_values(oc_dg *op, const double *values);
+int oc_dg_set_boundary_table(oc_dg *op, int32_t n_times, const double *times,
+                             const double *values);
 
 /* Geometry queries (host memory): cells [n_cells][4], degree and sizes {n_cells,
  * n_cells_padded, dofs_per_cell, nq, ng, n_inner, n_bdry}. */
@@ -177,6 +195,8 @@ int oc_dg_unpack(oc_dg *op, int32_t n_components, const double *op_order, double
  *   ki_q[0..n_ki-1]    vec_ki_discharge: [0] scratch (L_q), [stage+1] <- M^-1 L_q
  *   w_h, w_q           solution vectors; updated in place at the last stage
  *   r_h, r_q           next_ri vectors; written at the other stages
+ *   time               the stage time (current_time + c_s dt): selects the boundary data
+ *                      (oc_dg_set_boundary_table), OC_ERR_ARGUMENT if none is tabulated
  * ki_h / ki_q are host arrays of n_ki device pointers; n_ki = n_stages + 1 and the last
  * stage is stage == n_ki - 2 (OceanDG.h:1827).  The scratch entries ki_h[0], ki_q[0] are
  * written only if `write_scratch` is nonzero (the time integrator never reads them).
@@ -204,8 +224,8 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
  *   r_h, r_q                next_ri (written before the last stage; must differ from ri_h,
  *                           ri_q: the stage reads the input after writing r_h)
  * The last stage updates w_h, w_q in place and writes r_q = M^-1 (b . kex + b . kim).
- * Boundary data come from the stores set with oc_dg_set_boundary_values, so `time` is not
- * read.  ri_vel is read only when the viscosity is nonzero.  OC_PRECISION_MIXED: ki_h[1..]
+ * `time` selects the boundary data as in oc_dg_perform_stage_hydro.  ri_vel is read only
+ * when the viscosity is nonzero.  OC_PRECISION_MIXED: ki_h[1..]
  * and ki_q[1..] hold FP32 values in their first half as in the explicit stage (the explicit
  * and stiff momentum terms are FP32 integrals, so the storage is exact); the update that
  * combines them is FP64. */
@@ -248,13 +268,15 @@ int oc_dg_closure_source(oc_dg *op, const double *zeta, const double *q, double 
 
 /* dst = M^-1 L(src): step-67's OceanoOperator::apply (declared OceanDG.h:178-180; its
  * body in the reference is dead code).  Both fields use the exact tensor-product inverse
- * of the mass matrix (affine cells); L is the spatial operator of perform_stage_hydro. */
+ * of the mass matrix (affine cells); L is the spatial operator of perform_stage_hydro with
+ * the boundary data at `time` (OceanDG.h:1700-1705). */
 int oc_dg_apply(oc_dg *op, double time, const double *src_h, const double *src_q,
                 const double *src_vel, double *dst_h, double *dst_q, void *stream);
 
-/* L_zeta and L_q without inverse mass (the two MatrixFree::loop calls, OceanDG.h:1805-1823) */
-int oc_dg_rhs(oc_dg *op, const double *src_h, const double *src_q, const double *src_vel,
-              double *dst_h, double *dst_q, void *stream);
+/* L_zeta and L_q without inverse mass (the two MatrixFree::loop calls, OceanDG.h:1805-1823),
+ * boundary data at `time` */
+int oc_dg_rhs(oc_dg *op, double time, const double *src_h, const double *src_q,
+              const double *src_vel, double *dst_h, double *dst_q, void *stream);
 
 /* Nodal coefficients from values at the OC_STORE_CELL_COLLOC points (Gauss(p+1)^2 per
  * cell): transform_from_q_points_to_basis of deal.II's CellwiseInverseMassMatrix, as used
@@ -303,12 +325,23 @@ int oc_dg_sum(oc_dg *op, const double *x, int32_t n_components, double *sum, voi
  * the results are the same).  rows: S rows packed one after the other, row s holding
  * s+1 coefficients -- the Butcher row a[s+1][0..s] for s < S-1 and b for s = S-1
  * (unscaled; the step multiplies by dt).  A step is the S stages, then, when `velocity`
- * is given (viscosity != 0), the velocity update of the new state.  oc_rk_capture records
- * one step with fixed arguments into a HIP graph (on a non-null stream); oc_rk_replay
- * launches it n times.  Boundary data are whatever was tabulated before the capture. */
+ * is given (viscosity != 0), the velocity update of the new state.  The loop keeps its
+ * time t (oc_rk_set_time, initially 0): stage s of a step runs at t + c_s dt with c_s the
+ * row sum of a[s] (ExplicitRungeKuttaIntegrator.h:212-271), and every executed step
+ * advances t by dt.  oc_rk_capture records one step with fixed arguments into a HIP graph
+ * (on a non-null stream) without running it; oc_rk_replay launches it n times.  With
+ * time-dependent boundary data (oc_dg_set_boundary_table) the recorded stages read the
+ * table through a device-side entry counter that oc_rk_replay sets from t and the graph
+ * advances; the replay checks on the host that the table lists the n steps' stage times.
+ * A graph holds the operator's static data, boundary table and closure as they were at the
+ * capture: oc_rk_replay fails with OC_ERR_ARGUMENT after any oc_dg_set_static_data,
+ * oc_dg_set_boundary_values / _table, oc_dg_set_subgrid_stats or oc_dg_set_closure call on
+ * the operator (capture again). */
 typedef struct oc_rk oc_rk;
 int oc_rk_create(oc_dg *op, int32_t n_stages, const double *rows, oc_rk **out);
 void oc_rk_destroy(oc_rk *rk);
+int oc_rk_set_time(oc_rk *rk, double t);
+int oc_rk_time(const oc_rk *rk, double *t);
 int oc_rk_step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity, void *stream);
 int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity,
                   void *stream);

@@ -78,6 +78,8 @@ SIGNATURES = {
     "oc_rk_capture": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
     "oc_rk_replay": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "oc_rk_set_time": (C.c_int, [C.c_void_p, C.c_double]),
+    "oc_rk_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "oc_rk_registers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "oc_mesh_create_rectangle": (C.c_int, [D, D, D, D, I32, I32, I32, I32, PI32, C.POINTER(P)]),
@@ -93,6 +95,7 @@ SIGNATURES = {
     "oc_dg_quadrature_points": (C.c_int, [P, I32, PD]),
     "oc_dg_set_static_data": (C.c_int, [P, I32, PD]),
     "oc_dg_set_boundary_values": (C.c_int, [P, PD]),
+    "oc_dg_set_boundary_table": (C.c_int, [P, I32, PD, PD]),
     "oc_dg_sizes": (C.c_int, [P, PI64]),
     "oc_dg_vector_length": (C.c_int, [P, I32, PI64]),
     "oc_dg_pack": (C.c_int, [P, I32, P, P, P]),
@@ -103,7 +106,7 @@ SIGNATURES = {
                                                 C.POINTER(P), U32, C.POINTER(P), U32, P, P,
                                                 P, P, P]),
     "oc_dg_apply": (C.c_int, [P, D, P, P, P, P, P, P]),
-    "oc_dg_rhs": (C.c_int, [P, P, P, P, P, P, P]),
+    "oc_dg_rhs": (C.c_int, [P, D, P, P, P, P, P, P]),
     "oc_dg_count_nonfinite": (C.c_int, [P, P, I64, PI64, P]),
     "oc_dg_last_sweeps": (C.c_int, [P, PI64, P]),
     "oc_dg_project_collocation": (C.c_int, [P, I32, PD, PD]),

@@ -49,7 +49,15 @@ struct oc_dg
   double c_face = 0;
   double *d_bz = nullptr;
   double c_bz = 0;
-  double *d_bvals = nullptr;
+  // boundary data, one entry [n_bdry][nq][3] per tabulated time (oc_dg_set_boundary_table);
+  // bt_any: a single entry valid at every time (oc_dg_set_boundary_values)
+  double *d_btable = nullptr;
+  std::vector<double> bt_times;
+  bool bt_any = true;
+  bool needs_bdata = false;     // some boundary face reads boundary data
+  const int *capture_tick = nullptr;  // set while a time loop records a graph
+  // bumped by every setter that replaces device data a captured graph may hold
+  uint64_t generation = 0;
   unsigned int *d_sweeps_blk = nullptr;
   float *d_jmat = nullptr;  // mixed precision: closed-form Jacobi matrix (unit cell)
   unsigned long long *d_defer = nullptr;  // mixed precision: per batch, deferred lanes
@@ -84,7 +92,10 @@ struct oc_dg
     o.cell2n = {d_cell2n, c_cell2n};
     o.facez = {d_face, c_face};
     o.bdryz = {d_bz, c_bz};
-    o.bvals = d_bvals;
+    o.bvals = d_btable;
+    o.btick = nullptr;
+    o.bstage = 0;
+    o.bstride = (long long)(bdry.size() / 3) * nq * 3;
     o.bkind = d_bkind;
     o.g = g;
     o.cu4 = cu4;
@@ -155,6 +166,12 @@ enum TimedKernel
   TK_COUNT = 5
 };
 int timed_begin(oc_dg *op, hipStream_t s, hipEvent_t *start);
+// points O at the boundary data of a stage at `time` (or, while a time loop records a graph,
+// at the table indexed by the loop's device tick + stage); OC_ERR_ARGUMENT when the table
+// holds no entry at that time
+int boundary_entry(oc_dg *op, double time, uint32_t stage, OpArgs &O, const char *fn);
+// the entry of the boundary table at `time`, -1 if none (always 0 for time-independent data)
+long find_boundary_time(const oc_dg *op, double time);
 void timed_end(oc_dg *op, hipStream_t s, int kernel, hipEvent_t start);
 
 OC_DECLARE_DEGREE(1)

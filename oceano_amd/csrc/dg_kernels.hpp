@@ -70,7 +70,12 @@ struct OpArgs
   Field cell2n;         // [batch][n1^2][lane] friction coefficient at Gauss(n1) points
   Field facez;          // [face][ng][2]
   Field bdryz;          // [bface][nq]
-  const double *bvals;  // [bface][nq][3]
+  // boundary data [bface][nq][3] of this stage: the table entry itself, or (btick != null,
+  // a captured time loop) the table base, the entry being btick[0] + bstage
+  const double *bvals;
+  const int *btick;
+  int bstage;
+  long long bstride;    // doubles per table entry
   const int *bkind;     // [bface]
   double g, cu4, nu;
   int has_friction, has_wind, has_coriolis;
@@ -132,6 +137,12 @@ __device__ __forceinline__ void phase_barrier()
 // otherwise pairs each load with its use (in FP32 code with its conversion, to save
 // registers), so the wave waits out one memory latency per load instead of one per group.
 __device__ __forceinline__ void issue_barrier() { __builtin_amdgcn_sched_barrier(0); }
+
+// boundary data of the running stage (see OpArgs::bvals)
+__device__ __forceinline__ const double *stage_bvals(const OpArgs &O)
+{
+  return O.btick ? O.bvals + (size_t)(O.btick[0] + O.bstage) * (size_t)O.bstride : O.bvals;
+}
 
 __device__ __forceinline__ double fld(const Field &f, size_t i)
 {

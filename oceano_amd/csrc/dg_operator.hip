@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -379,7 +380,7 @@ void free_all(oc_dg *op)
   f(op->d_closure);
   f(op->d_face);
   f(op->d_bz);
-  f(op->d_bvals);
+  f(op->d_btable);
   f(op->d_sweeps_blk);
   f(op->d_jmat);
   f(op->d_defer);
@@ -485,6 +486,44 @@ int cell_slots(const oc_mesh_desc *mesh, int precision, std::vector<int32_t> &or
 }
 
 }  // namespace
+
+namespace oceano {
+
+long find_boundary_time(const oc_dg *op, double time)
+{
+  if (op->bt_any)
+    return 0;
+  const double tol = 1e-12 * std::max(1.0, std::fabs(time));
+  for (size_t k = 0; k < op->bt_times.size(); ++k)
+    if (std::fabs(op->bt_times[k] - time) <= tol)
+      return (long)k;
+  return -1;
+}
+
+int boundary_entry(oc_dg *op, double time, uint32_t stage, OpArgs &O, const char *fn)
+{
+  if (!op->needs_bdata)
+    return OC_OK;
+  if (op->capture_tick)
+    {
+      O.btick = op->capture_tick;
+      O.bstage = (int)stage;
+      return OC_OK;
+    }
+  const long k = find_boundary_time(op, time);
+  if (k < 0)
+    {
+      char buf[64];
+      std::snprintf(buf, sizeof(buf), "%.17g", time);
+      set_error(std::string(fn) + ": no boundary data tabulated at time " + buf +
+                " (oc_dg_set_boundary_table)");
+      return OC_ERR_ARGUMENT;
+    }
+  O.bvals = op->d_btable + (size_t)k * (size_t)O.bstride;
+  return OC_OK;
+}
+
+}  // namespace oceano
 
 extern "C" {
 
@@ -673,6 +712,8 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
                       ", did you set a boundary condition for this part of the domain boundary?",
                     OC_ERR_BOUNDARY);
       bkind[b] = it->second;
+      if (it->second != OC_BC_WALL && it->second != OC_BC_SUPERCRITICAL_OUTFLOW)
+        op->needs_bdata = true;
       if (!set_kind(c, f, FK_BDRY))
         return fail("boundary face " + std::to_string(b) + " overlaps an inner face", OC_ERR_MESH);
       fid[at8(c, f, 0)] = b;
@@ -719,7 +760,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
   if (mesh->n_boundary > 0)
     {
       std::vector<double> bv((size_t)mesh->n_boundary * op->nq * 3, 0.0);
-      if ((rc = upload(&op->d_bvals, bv)))
+      if ((rc = upload(&op->d_btable, bv)))
         {
           free_all(op);
           delete op;
@@ -874,6 +915,9 @@ int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
   int64_t npts = 0;
   if (const int rc = oc_dg_n_points(op, store, &npts))
     return rc;
+  // the stores are reallocated (or turned into constants): a graph recorded before holds
+  // the old pointers and values
+  op->generation++;
   if (npts == 0)
     return OC_OK;  // empty store: nothing is read from it
   if (!values)
@@ -977,6 +1021,19 @@ int oc_dg_set_static_data(oc_dg *op, int32_t store, const double *values)
     }
 }
 
+
+static int upload_boundary(oc_dg *op, size_t n_times, const double *values)
+{
+  const size_t n = (op->bdry.size() / 3) * op->nq * 3;
+  // a stage still running on another stream may read the current table
+  HIP_TRY(hipDeviceSynchronize());
+  std::vector<double> v(values, values + n * n_times);
+  if (const int rc = upload(&op->d_btable, v))
+    return rc;
+  op->generation++;
+  return OC_OK;
+}
+
 int oc_dg_set_boundary_values(oc_dg *op, const double *values)
 {
   if (!op)
@@ -986,7 +1043,34 @@ int oc_dg_set_boundary_values(oc_dg *op, const double *values)
     return OC_OK;
   if (!values)
     return bad_argument(__func__);
-  HIP_TRY(hipMemcpy(op->d_bvals, values, n * sizeof(double), hipMemcpyHostToDevice));
+  if (const int rc = upload_boundary(op, 1, values))
+    return rc;
+  op->bt_times.clear();
+  op->bt_any = true;
+  return OC_OK;
+}
+
+int oc_dg_set_boundary_table(oc_dg *op, int32_t n_times, const double *times,
+                             const double *values)
+{
+  if (!op || n_times < 1 || !times)
+    return bad_argument(__func__);
+  for (int32_t k = 0; k < n_times; ++k)
+    if (!std::isfinite(times[k]) || (k > 0 && times[k] < times[k - 1]))
+      {
+        set_error("oc_dg_set_boundary_table: times must be finite and non-decreasing");
+        return OC_ERR_ARGUMENT;
+      }
+  const size_t n = (op->bdry.size() / 3) * op->nq * 3;
+  if (n > 0)
+    {
+      if (!values)
+        return bad_argument(__func__);
+      if (const int rc = upload_boundary(op, (size_t)n_times, values))
+        return rc;
+    }
+  op->bt_times.assign(times, times + n_times);
+  op->bt_any = false;
   return OC_OK;
 }
 
@@ -1020,7 +1104,6 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
                               uint32_t n_ki, double *w_h, double *w_q, double *r_h,
                               double *r_q, int32_t write_scratch, void *stream)
 {
-  (void)time;
   if (!op || !factor || !ri_h || !ri_q || !ki_h || !ki_q || !w_h || !w_q)
     {
       set_error("oc_dg_perform_stage_hydro: null argument");
@@ -1051,6 +1134,8 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
         return OC_ERR_ARGUMENT;
       }
   OpArgs O = op->args();
+  if (const int rc = boundary_entry(op, time, stage, O, "oc_dg_perform_stage_hydro"))
+    return rc;
   hipStream_t s = (hipStream_t)stream;
   if (op->closure_mlp)
     {
@@ -1076,17 +1161,20 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
     }
 }
 
-static int rhs_or_apply(oc_dg *op, bool invert, const double *src_h, const double *src_q,
-                        const double *src_vel, double *dst_h, double *dst_q, void *stream)
+static int rhs_or_apply(oc_dg *op, bool invert, double time, const double *src_h,
+                        const double *src_q, const double *src_vel, double *dst_h,
+                        double *dst_q, void *stream, const char *fn)
 {
   if (!op || !src_h || !src_q || !dst_h || !dst_q)
-    return bad_argument(__func__);
+    return bad_argument(fn);
   if (op->nu != 0.0 && !src_vel)
     {
       set_error("velocity vector required when viscosity != 0");
       return OC_ERR_ARGUMENT;
     }
-  const OpArgs O = op->args();
+  OpArgs O = op->args();
+  if (const int rc = boundary_entry(op, time, 0, O, fn))
+    return rc;
   hipStream_t s = (hipStream_t)stream;
   switch (op->P)
     {
@@ -1109,7 +1197,6 @@ int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
                                   uint32_t n_ki_q, double *w_h, double *w_q, double *r_h,
                                   double *r_q, void *stream)
 {
-  (void)time;  // boundary data are tabulated by the caller (oc_dg_set_boundary_values)
   if (!op || !factor || !ri_h || !ri_q || !ki_h || !ki_q || !w_h || !w_q || !r_q)
     {
       set_error("oc_dg_perform_stage_hydro_ark: null argument");
@@ -1136,7 +1223,9 @@ int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
         set_error("oc_dg_perform_stage_hydro_ark: null ki vector");
         return OC_ERR_ARGUMENT;
       }
-  const OpArgs O = op->args();
+  OpArgs O = op->args();
+  if (const int rc = boundary_entry(op, time, stage, O, "oc_dg_perform_stage_hydro_ark"))
+    return rc;
   hipStream_t s = (hipStream_t)stream;
   switch (op->P)
     {
@@ -1158,14 +1247,15 @@ int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,
 int oc_dg_apply(oc_dg *op, double time, const double *src_h, const double *src_q,
                 const double *src_vel, double *dst_h, double *dst_q, void *stream)
 {
-  (void)time;
-  return rhs_or_apply(op, true, src_h, src_q, src_vel, dst_h, dst_q, stream);
+  return rhs_or_apply(op, true, time, src_h, src_q, src_vel, dst_h, dst_q, stream,
+                      "oc_dg_apply");
 }
 
-int oc_dg_rhs(oc_dg *op, const double *src_h, const double *src_q, const double *src_vel,
-              double *dst_h, double *dst_q, void *stream)
+int oc_dg_rhs(oc_dg *op, double time, const double *src_h, const double *src_q,
+              const double *src_vel, double *dst_h, double *dst_q, void *stream)
 {
-  return rhs_or_apply(op, false, src_h, src_q, src_vel, dst_h, dst_q, stream);
+  return rhs_or_apply(op, false, time, src_h, src_q, src_vel, dst_h, dst_q, stream,
+                      "oc_dg_rhs");
 }
 
 int oc_dg_project_collocation(const oc_dg *op, int32_t n_comp, const double *values,
@@ -1402,6 +1492,7 @@ int oc_dg_set_subgrid_stats(oc_dg *op, const double *stats, const int32_t *activ
 {
   if (!op || (!stats && op->N > 0))
     return bad_argument(__func__);
+  op->generation++;
   if (3 * op->nd + 3 > mlp_inputs_max)
     {
       set_error("oc_dg_set_subgrid_stats: the closure supports degrees 1 and 2");
@@ -1446,6 +1537,7 @@ int oc_dg_set_closure(oc_dg *op, const oc_mlp *mlp)
 {
   if (!op)
     return bad_argument(__func__);
+  op->generation++;
   if (!mlp)
     {
       op->closure_mlp = nullptr;

@@ -9,6 +9,8 @@
 // launch overhead from the time loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -24,15 +26,24 @@ struct oc_rk
   oc_dg *op = nullptr;
   int S = 0;
   std::vector<std::vector<double>> rows;  // unscaled: row s feeds stage s's update
+  std::vector<double> c;                  // stage s runs at t + c[s] dt (row sums)
   std::vector<double *> kh, kq;           // S + 1 each
   double *rh[2] = {nullptr, nullptr}, *rq[2] = {nullptr, nullptr};
+  double t = 0.0;                         // time of the next step
+  int *d_tick = nullptr;                  // boundary-table entry of the replayed step
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  double graph_dt = 0.0;
+  bool graph_ticks = false;               // the graph indexes the table by d_tick
+  uint64_t graph_generation = 0;
 };
 
 using oceano::set_error;
 
 namespace {
+
+__global__ void k_set_tick(int *tick, int v) { *tick = v; }
+__global__ void k_add_tick(int *tick, int v) { *tick += v; }
 
 void release(oc_rk *rk)
 {
@@ -51,6 +62,7 @@ void release(oc_rk *rk)
       (void)hipFree(rk->rh[i]);
       (void)hipFree(rk->rq[i]);
     }
+  (void)hipFree(rk->d_tick);
 }
 
 int zeros(double **p, size_t n)
@@ -64,6 +76,8 @@ int zeros(double **p, size_t n)
   return OC_OK;
 }
 
+// the S stages of one step at time rk->t (ExplicitRungeKuttaIntegrator.h:212-271: stage s
+// at current_time + c_s time_step)
 int step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity, hipStream_t s)
 {
   const int S = rk->S;
@@ -75,9 +89,9 @@ int step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity, hipSt
       const double *ri_h = st == 0 ? w_h : rk->rh[st % 2];
       const double *ri_q = st == 0 ? w_q : rk->rq[st % 2];
       const int rc = oc_dg_perform_stage_hydro(
-        rk->op, (uint32_t)st, 0.0, f.data(), (uint32_t)st + 1, ri_h, ri_q, velocity,
-        rk->kh.data(), rk->kq.data(), (uint32_t)S + 1, w_h, w_q, rk->rh[(st + 1) % 2],
-        rk->rq[(st + 1) % 2], 0, s);
+        rk->op, (uint32_t)st, rk->t + rk->c[st] * dt, f.data(), (uint32_t)st + 1, ri_h, ri_q,
+        velocity, rk->kh.data(), rk->kq.data(), (uint32_t)S + 1, w_h, w_q,
+        rk->rh[(st + 1) % 2], rk->rq[(st + 1) % 2], 0, s);
       if (rc)
         return rc;
     }
@@ -101,9 +115,17 @@ int oc_rk_create(oc_dg *op, int32_t n_stages, const double *rows, oc_rk **out)
   rk->op = op;
   rk->S = n_stages;
   size_t o = 0;
+  rk->c.assign(1, 0.0);
   for (int s = 0; s < n_stages; ++s)
     {
       rk->rows.emplace_back(rows + o, rows + o + s + 1);
+      if (s + 1 < n_stages)
+        {
+          double cs = 0.0;
+          for (int j = 0; j <= s; ++j)
+            cs += rows[o + j];
+          rk->c.push_back(cs);
+        }
       o += (size_t)s + 1;
     }
   const size_t n1 = (size_t)op->NB * oceano::CB * op->nd, n2 = 2 * n1;
@@ -114,6 +136,11 @@ int oc_rk_create(oc_dg *op, int32_t n_stages, const double *rows, oc_rk **out)
     rc = zeros(&rk->kh[j], n1) ? OC_ERR_HIP : zeros(&rk->kq[j], n2);
   for (int i = 0; i < 2 && !rc; ++i)
     rc = zeros(&rk->rh[i], n1) ? OC_ERR_HIP : zeros(&rk->rq[i], n2);
+  if (!rc && hipMalloc((void **)&rk->d_tick, sizeof(int)) != hipSuccess)
+    {
+      set_error("oc_rk_create: device allocation failed");
+      rc = OC_ERR_HIP;
+    }
   if (rc)
     {
       release(rk);
@@ -139,7 +166,32 @@ int oc_rk_step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity,
       set_error("oc_rk_step: null argument");
       return OC_ERR_ARGUMENT;
     }
-  return step(rk, dt, w_h, w_q, velocity, (hipStream_t)stream);
+  const int rc = step(rk, dt, w_h, w_q, velocity, (hipStream_t)stream);
+  if (!rc)
+    rk->t += dt;
+  return rc;
+}
+
+int oc_rk_set_time(oc_rk *rk, double t)
+{
+  if (!rk || !std::isfinite(t))
+    {
+      set_error("oc_rk_set_time: null handle or non-finite time");
+      return OC_ERR_ARGUMENT;
+    }
+  rk->t = t;
+  return OC_OK;
+}
+
+int oc_rk_time(const oc_rk *rk, double *t)
+{
+  if (!rk || !t)
+    {
+      set_error("oc_rk_time: null argument");
+      return OC_ERR_ARGUMENT;
+    }
+  *t = rk->t;
+  return OC_OK;
 }
 
 int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity,
@@ -158,17 +210,34 @@ int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *veloci
   rk->exec = nullptr;
   rk->graph = nullptr;
   // per-kernel event timing is not recorded into the graph
-  const int timing = rk->op->timing;
-  rk->op->timing = 0;
+  oc_dg *op = rk->op;
+  const int timing = op->timing;
+  op->timing = 0;
+  // time-dependent boundary data: the recorded stages index the table by the device tick,
+  // which oc_rk_replay sets to the entry of the first replayed stage and the graph advances
+  // by S at the end of every step
+  const bool ticks = op->needs_bdata && !op->bt_any;
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
     {
-      rk->op->timing = timing;
+      op->timing = timing;
       set_error("oc_rk_capture: hipStreamBeginCapture failed");
       return OC_ERR_HIP;
     }
-  const int rc = step(rk, dt, w_h, w_q, velocity, s);
+  if (ticks)
+    op->capture_tick = rk->d_tick;
+  int rc = step(rk, dt, w_h, w_q, velocity, s);
+  op->capture_tick = nullptr;
+  if (!rc && ticks)
+    {
+      hipLaunchKernelGGL(k_add_tick, dim3(1), dim3(1), 0, s, rk->d_tick, rk->S);
+      if (hipGetLastError() != hipSuccess)
+        {
+          set_error("oc_rk_capture: tick kernel launch failed");
+          rc = OC_ERR_HIP;
+        }
+    }
   const hipError_t e = hipStreamEndCapture(s, &rk->graph);
-  rk->op->timing = timing;
+  op->timing = timing;
   if (rc)
     return rc;
   if (e != hipSuccess ||
@@ -177,6 +246,9 @@ int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *veloci
       set_error("oc_rk_capture: graph capture or instantiation failed");
       return OC_ERR_HIP;
     }
+  rk->graph_dt = dt;
+  rk->graph_ticks = ticks;
+  rk->graph_generation = op->generation;
   return OC_OK;
 }
 
@@ -187,12 +259,48 @@ int oc_rk_replay(oc_rk *rk, int32_t n, void *stream)
       set_error("oc_rk_replay: no captured step");
       return OC_ERR_ARGUMENT;
     }
+  oc_dg *op = rk->op;
+  if (op->generation != rk->graph_generation)
+    {
+      set_error("oc_rk_replay: the operator's static data, boundary data or closure changed "
+                "since oc_rk_capture; capture again");
+      return OC_ERR_ARGUMENT;
+    }
+  hipStream_t s = (hipStream_t)stream;
+  if (rk->graph_ticks && n > 0)
+    {
+      // the n steps read consecutive table entries from the one at the current time
+      const long e0 = oceano::find_boundary_time(op, rk->t);
+      const double dt = rk->graph_dt;
+      bool ok = e0 >= 0 && (size_t)e0 + (size_t)n * rk->S <= op->bt_times.size();
+      for (int r = 0; ok && r < n; ++r)
+        for (int st = 0; ok && st < rk->S; ++st)
+          {
+            const double t = rk->t + r * dt + rk->c[st] * dt;
+            const double tab = op->bt_times[(size_t)e0 + (size_t)r * rk->S + st];
+            ok = std::fabs(tab - t) <= 1e-12 * std::max(1.0, std::fabs(t));
+          }
+      if (!ok)
+        {
+          set_error("oc_rk_replay: the boundary table does not list the stage times of the "
+                    "replayed steps consecutively from the current time (entries "
+                    "t + r dt + c_s dt, r = 0..n-1, s = 0..S-1)");
+          return OC_ERR_ARGUMENT;
+        }
+      hipLaunchKernelGGL(k_set_tick, dim3(1), dim3(1), 0, s, rk->d_tick, (int)e0);
+      if (hipGetLastError() != hipSuccess)
+        {
+          set_error("oc_rk_replay: tick kernel launch failed");
+          return OC_ERR_HIP;
+        }
+    }
   for (int i = 0; i < n; ++i)
-    if (hipGraphLaunch(rk->exec, (hipStream_t)stream) != hipSuccess)
+    if (hipGraphLaunch(rk->exec, s) != hipSuccess)
       {
         set_error("oc_rk_replay: hipGraphLaunch failed");
         return OC_ERR_HIP;
       }
+  rk->t += n * rk->graph_dt;
   return OC_OK;
 }
 

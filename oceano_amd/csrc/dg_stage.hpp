@@ -275,6 +275,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
           const Vec2<R> n = {f == 0 ? R(-1) : f == 1 ? R(1) : R(0),
                              f == 2 ? R(-1) : f == 3 ? R(1) : R(0)};
           const FieldV<R> bz = in_vgprs<R>(O.bdryz);
+          const double *bvb = stage_bvals(O);
 #pragma unroll 1
           for (int a = 0; a < NQ; ++a)
             {
@@ -288,7 +289,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                 }
               zo += zcr;
               const R zb = fld(bz, (size_t)bf * NQ + a);
-              const double *bv = O.bvals + ((size_t)bf * NQ + a) * 3;
+              const double *bv = bvb + ((size_t)bf * NQ + a) * 3;
               R val[NC];
               bdry(bk, zo, Vec2<R>{xo, yo}, n, zb, bv, val);
               const R JxW = T.wq[a] * flen;

@@ -189,6 +189,18 @@ class DeviceIntegrator:
     def _p(t):
         return None if t is None else t.data_ptr()
 
+    def set_time(self, t):
+        """time of the next step (stage s runs at t + c_s dt; every step advances it)"""
+        from . import _capi
+        _capi.check(_capi.lib().oc_rk_set_time(self._h, float(t)), "oc_rk_set_time")
+
+    def time(self):
+        import ctypes as C
+        from . import _capi
+        t = C.c_double()
+        _capi.check(_capi.lib().oc_rk_time(self._h, C.byref(t)), "oc_rk_time")
+        return t.value
+
     def step(self, dt, w_h, w_q, velocity=None):
         from . import _capi
         from .operator import _stream

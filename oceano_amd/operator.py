@@ -84,8 +84,22 @@ class OceanoOperator:
         check(lib().oc_dg_quadrature_points(self._h, s, dptr(xy)), "oc_dg_quadrature_points")
         return xy
 
+    # values per point of each store (OC_STORE_* in oceano_dg.h)
+    VALUES_PER_POINT = {"cell_0": 5, "cell_1": 1, "cell_2": 2, "face": 1, "boundary": 1}
+
+    def n_points(self, store):
+        n = np.zeros(1, np.int64)
+        check(lib().oc_dg_n_points(self._h, STORES[store], n.ctypes.data_as(_capi.PI64)),
+              "oc_dg_n_points")
+        return int(n[0])
+
     def set_static_data(self, store, values):
         v = np.ascontiguousarray(values, dtype=np.float64)
+        expect = self.n_points(store) * self.VALUES_PER_POINT[store]
+        if v.size != expect:
+            raise ValueError("%s: %d values given, the store holds %d points x %d values"
+                             % (store, v.size, self.n_points(store),
+                                self.VALUES_PER_POINT[store]))
         check(lib().oc_dg_set_static_data(self._h, STORES[store], dptr(v)),
               "oc_dg_set_static_data")
 
@@ -102,8 +116,27 @@ class OceanoOperator:
         return data
 
     def set_boundary_values(self, values):
+        """time-independent boundary data [n_boundary, nq, 3] (used at every time)"""
         v = np.ascontiguousarray(values, dtype=np.float64)
+        expect = self.n_boundary * self.n_q_points_1d * 3
+        if v.size != expect:
+            raise ValueError("boundary values: %d given, expected n_boundary x nq x 3 = %d"
+                             % (v.size, expect))
         check(lib().oc_dg_set_boundary_values(self._h, dptr(v)), "oc_dg_set_boundary_values")
+
+    def set_boundary_table(self, times, values):
+        """boundary data at the given times, [n_times, n_boundary, nq, 3]: every call with a
+        time uses the entry at that time (OceanDG.h:1798-1803 sets the boundary Functions'
+        time each stage); see oc_dg_set_boundary_table for the order a replayed time loop
+        reads"""
+        t = np.ascontiguousarray(times, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        expect = t.size * self.n_boundary * self.n_q_points_1d * 3
+        if v.size != expect:
+            raise ValueError("boundary table: %d values given, expected n_times x n_boundary "
+                             "x nq x 3 = %d" % (v.size, expect))
+        check(lib().oc_dg_set_boundary_table(self._h, t.size, dptr(t), dptr(v)),
+              "oc_dg_set_boundary_table")
 
     # --------------------------------------------------------------- vectors -------
     def vector_length(self, n_components):
@@ -200,11 +233,11 @@ class OceanoOperator:
         check(lib().oc_dg_apply(self._h, current_time, _vp(src[0]), _vp(src[1]), _vp(vel),
                                 _vp(dst[0]), _vp(dst[1]), _stream()), "oc_dg_apply")
 
-    def rhs(self, src, dst):
+    def rhs(self, src, dst, current_time=0.0):
         """dst = L(src) without inverse mass (the two MatrixFree::loop calls)."""
         vel = src[2] if len(src) > 2 else None
-        check(lib().oc_dg_rhs(self._h, _vp(src[0]), _vp(src[1]), _vp(vel), _vp(dst[0]),
-                              _vp(dst[1]), _stream()), "oc_dg_rhs")
+        check(lib().oc_dg_rhs(self._h, current_time, _vp(src[0]), _vp(src[1]), _vp(vel),
+                              _vp(dst[0]), _vp(dst[1]), _stream()), "oc_dg_rhs")
 
     def compute_errors(self, zeta, q, exact, time):
         """OceanDG.h:2382-2459 per field: {depth, discharge, hu, hv} L2 and Linf errors of

@@ -54,7 +54,11 @@ class Problem:
 
 def vortex(degree=1, refine=3, refine_fraction=None, seed=20261202, periodic=True,
            u_inf=(1.0, 0.0), amplitude=0.1, radius=0.35, center=(0.5, 0.5), h0=1.0,
-           g=9.81, nx=20, ny=10):
+           g=9.81, nx=20, ny=10, bcs=None):
+    """Travelling vortex; non-periodic: boundary ids 1 (x = 0), 2 (x = 2), 0 (y = 0, 1) with
+    the reference's supercritical inflow / outflow / wall conditions unless `bcs` maps the
+    ids to other kinds.  Boundary data are the exact solution at the stage time (the
+    reference's ExactSolution boundary functions, Icbc_ShallowWaterVortex.h:262-271)."""
     mesh = Mesh.rectangle(0.0, 2.0, 0.0, 1.0, nx, ny, periodic=(periodic, periodic),
                           boundary_ids=(1, 2, 0, 0))
     if refine:
@@ -70,7 +74,10 @@ def vortex(degree=1, refine=3, refine_fraction=None, seed=20261202, periodic=Tru
     def problem_data(xy):
         return np.zeros((xy.shape[0], 5))
 
-    bcs = {} if periodic else {0: "wall", 1: "supercritical_inflow", 2: "supercritical_outflow"}
+    if periodic:
+        bcs = {}
+    elif bcs is None:
+        bcs = {0: "wall", 1: "supercritical_inflow", 2: "supercritical_outflow"}
 
     def boundary_values(op, t):
         xy = op.quadrature_points("boundary")
@@ -105,6 +112,44 @@ def channel(degree=2, nx=250, ny=10, seed=7, manning=0.02, discharge=4.42, h_out
                    name="channel_p%d" % degree)
 
 
+def forced(degree=2, refine=1, refine_fraction=0.3, seed=5, uniform=False, g=9.81):
+    """A periodic basin driven by wind stress and Coriolis over a smooth bed with Manning
+    friction: every term of ShallowWater::source (ShallowWater.h:408-432) is active.
+    uniform=False stores non-uniform wind and Coriolis fields (a beta plane f = f0 + beta y)
+    and a non-flat bed; uniform=True makes every problem-data field a constant (the kernels
+    then read each as a scalar)."""
+    mesh = Mesh.rectangle(0.0, 2.0, 0.0, 1.0, 10, 5, periodic=(True, True))
+    if refine:
+        mesh.refine_global(refine)
+    if refine_fraction:
+        mesh.refine_fraction(refine_fraction, seed)
+
+    def problem_data(xy):
+        x, y = xy[:, 0], xy[:, 1]
+        out = np.zeros((xy.shape[0], 5))
+        if uniform:
+            out[:, 0] = -0.1
+            out[:, 1] = 0.03
+            out[:, 2] = 0.06
+            out[:, 3] = -0.04
+            out[:, 4] = 0.9
+        else:
+            out[:, 0] = -0.1 + 0.05 * np.sin(np.pi * x) * np.cos(2 * np.pi * y)
+            out[:, 1] = 0.03
+            out[:, 2] = 0.05 + 0.02 * np.sin(2 * np.pi * y)
+            out[:, 3] = -0.03 * np.cos(np.pi * x)
+            out[:, 4] = 0.5 + 0.3 * np.sin(2 * np.pi * y)
+        return out
+
+    def initial(xy):
+        x, y = xy[:, 0], xy[:, 1]
+        return np.stack([1.0 + 0.05 * np.sin(np.pi * x) * np.sin(2 * np.pi * y),
+                         0.3 + 0.1 * np.cos(2 * np.pi * y), 0.1 * np.sin(np.pi * x)], 1)
+
+    return Problem(mesh, degree, {}, problem_data, initial, None, None, g=g,
+                   name="forced_%s_p%d" % ("uniform" if uniform else "stored", degree))
+
+
 def lake_at_rest(degree=2, nx=8, ny=4, level=1.0, seed=3):
     """zeta = const, q = 0 over a bed with smooth and discontinuous parts
     (Icbc_LakeAtRest.h:45-77): the well-balanced operator returns L = 0."""
@@ -126,6 +171,15 @@ def lake_at_rest(degree=2, nx=8, ny=4, level=1.0, seed=3):
 
     return Problem(mesh, degree, {0: "wall"}, problem_data, initial, None, None,
                    name="lake_at_rest")
+
+
+def boundary_table(problem, op, t0, dt, n_steps, c):
+    """Boundary data of n_steps RK steps from t0 in the order a replayed time loop reads
+    them: entry r S + s holds the data at t0 + r dt + c[s] dt (the stage times of
+    ExplicitRungeKuttaIntegrator.h:212-271).  Returns (times, values[n, n_bdry, nq, 3])."""
+    times = np.array([t0 + r * dt + cs * dt for r in range(n_steps) for cs in c])
+    values = np.stack([problem.boundary_values(op, t) for t in times])
+    return times, values
 
 
 def setup_operator(problem, precision="fp64", device="cuda", max_iteration_height=15):

@@ -1782,4 +1782,30 @@ double ocr_volume(void *h, const double *zeta)
   return s;
 }
 
+// The exterior state of one boundary quadrature point (OceanDG.h:1130-1203): kind as in
+// bc_kind, interior state (z_m, q_m), outward normal n, bed z_b, boundary data bv[3].
+// out = {z_p, qx_p, qy_p}; returns 1 at a supercritical outflow (where the momentum flux
+// is cut under backflow, OceanDG.h:1326-1332), 0 otherwise, -1 for an unknown kind.
+int ocr_boundary_state(void *h, int kind, double z_m, double qx_m, double qy_m, double nx,
+                       double ny, double zb, const double *bv, double *out)
+{
+  Oracle *o = static_cast<Oracle *>(h);
+  try
+    {
+      double z_p;
+      V2 q_p;
+      const bool at_outflow = o->boundary_state(kind, z_m, {qx_m, qy_m}, {nx, ny}, zb, bv, z_p,
+                                                q_p);
+      out[0] = z_p;
+      out[1] = q_p.x;
+      out[2] = q_p.y;
+      return at_outflow ? 1 : 0;
+    }
+  catch (const std::string &e)
+    {
+      g_err = e;
+      return -1;
+    }
+}
+
 }  // extern "C"

@@ -57,6 +57,9 @@ def lib():
                                                       C.POINTER(PD), PD, PD, PD, PD]),
             "ocr_transport_speed": (C.c_double, [C.c_void_p, PD, PD]),
             "ocr_volume": (C.c_double, [C.c_void_p, PD]),
+            "ocr_boundary_state": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                             C.c_double, C.c_double, C.c_double, C.c_double,
+                                             PD, PD]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -240,6 +243,18 @@ class Oracle:
     def volume(self, zeta):
         """check_mass cell integral (OceanDG.h:2138-2184): integral of the depth"""
         return lib().ocr_volume(self._h, _d(zeta))
+
+    def boundary_state(self, kind, z_m, q_m, n, zb, bv):
+        """exterior state (z_p, q_p) of one boundary point and the outflow flag
+        (OceanDG.h:1130-1203)"""
+        k = BC[kind] if isinstance(kind, str) else int(kind)
+        b = np.ascontiguousarray(bv, np.float64)
+        out = np.zeros(3)
+        rc = lib().ocr_boundary_state(self._h, k, z_m, q_m[0], q_m[1], n[0], n[1], zb, _d(b),
+                                      _d(out))
+        if rc < 0:
+            raise RuntimeError(lib().ocr_last_error().decode())
+        return out[0], out[1:].copy(), bool(rc)
 
     def perform_stage_hydro_ark(self, stage, n_stages, factor, factor_tilde, fm, ri_h, ri_q,
                                 ri_v, ki_h, ki_q, w_h, w_q, r_h, r_q):
