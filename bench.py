@@ -6,15 +6,19 @@ globally 4 times, then 30% of the cells (std::mt19937_64 seed 20261202) refined 
 97,280 cells with 85,944 hanging subfaces, 4.67 M scalar DoF (zeta, hu, hv), periodic.
 A "step" is one explicit RK stage (oc_dg_perform_stage_hydro) of classic RK4 over the
 whole mesh; inputs are resident in HBM when timing starts.  The metric counts
-DoF x stages / second.  The timed region runs the device-resident time loop: one RK4 step
-captured into a HIP graph and replayed (plus eager stages for a remainder), exactly
---steps stages between two synchronisations.
+DoF x stages / second.  The timed region runs the device-resident time loop: the whole RK4
+steps of the region captured into one HIP graph (launched once before, untimed) and
+replayed once, plus eager stages for a remainder: exactly --steps stages between two
+synchronisations, timed by HIP events on the replay stream (`value`; the host clock around
+the same region gives `wall_value`).
 
 Other workloads (--workload): vortex_p1_160x80 (12,800 cells, p = 1, cache-resident),
 vortex_p1_819k (the same refined to 819,200 cells, 9.8 M DoF), channel_p2_250x10 (2,500
-cells, p = 2, discharge inflow / height outflow / walls / Manning / bumpy bed) and
-ensemble_64 (64 vortex instances of 1k-50k cells and degrees 1-4, each degree's instances
-advanced by one operator over the union of their meshes; 25.6 M DoF).
+cells, p = 2, discharge inflow / height outflow / walls / Manning / bumpy bed),
+channel_p2_640k (the same channel at 4000 x 160 cells, 17.3 M DoF: the bed is stored at
+every quadrature point and streamed every stage) and ensemble_64 (64 vortex instances of
+1k-50k cells and degrees 1-4, each degree's instances advanced by one operator over the
+union of their meshes; 25.6 M DoF).
 
 Extra fields: `roofline` for the kernel that takes most of the stage time (algorithmic
 HBM bytes per launch / mean launch time from HIP events recorded on the launch stream
@@ -39,10 +43,48 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 DEFAULT = "vortex_p3_hanging_97k"
-# HBM bytes per launch of the stage kernel on the default workload, measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 512 MiB copy
-# (profiles/k_stage_vortex_p3_hanging_97k.md)
-MEASURED_TRAFFIC = {DEFAULT: {"fp64": 277.5e6, "mixed": 240.2e6}}
+WORKLOADS = [DEFAULT, "vortex_p1_160x80", "vortex_p1_819k", "channel_p2_250x10",
+             "channel_p2_640k", "ensemble_64"]
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the stage kernel, per precision, as the committed profile
+    summary of the workload states them (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a
+    copy kernel of known size): its line 'traffic per launch (bytes): fp64 N mixed M'.
+    Returns ({precision: bytes}, path) or ({}, None) when no profile holds the figure."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_stage_%s.md" % workload)
+    if not os.path.exists(path):
+        return {}, None
+    for line in open(path):
+        m = re.search(r"traffic per launch \(bytes\):\s*fp64\s+([0-9.e+]+)\s+mixed\s+([0-9.e+]+)",
+                      line)
+        if m:
+            return {"fp64": float(m.group(1)), "mixed": float(m.group(2))}, \
+                os.path.relpath(path, ROOT)
+    return {}, None
+
+
+def static_bytes(op, data):
+    """HBM bytes of the static stores one stage reads (OceanDG.h:498-569 read every stage):
+    each stored field once (a field whose values are all equal is a kernel constant and
+    costs nothing); z_b of cell_0 per Gauss point (discharge work), z_b of cell_1 per
+    Gauss-Lobatto point (depth solve), the face z_b pair per face point, the boundary z_b
+    and boundary data per boundary point."""
+    def stored(v):
+        v = np.asarray(v)
+        return v.size > 0 and not np.all(v == v.reshape(-1)[0])
+    b = 0
+    c0 = np.asarray(data["cell_0"])
+    for f in range(5):
+        if stored(c0[:, f]):
+            b += 8 * c0.shape[0]
+    for k in ("cell_1", "face", "boundary"):
+        if stored(data[k]):
+            b += 8 * np.asarray(data[k]).size
+    if op.n_boundary:
+        b += 8 * 3 * op.n_boundary * op.n_q_points_1d
+    return b
 
 
 def workload(name):
@@ -59,6 +101,9 @@ def workload(name):
     if name == "channel_p2_250x10":  # the channel deck's CFL
         return [problems.channel(degree=2, nx=250, ny=10)], [("cfl", 0.075)], 2, \
             "cache-resident (0.5 MB per state)"
+    if name == "channel_p2_640k":  # the channel refined 4 times: 17.3 M DoF, stored bed
+        return [problems.channel(degree=2, nx=4000, ny=160)], [("cfl", 0.075)], 2, \
+            "HBM-resident; bed stored at every quadrature point (static data streamed)"
     if name == "ensemble_64":  # CFL 0.15 / (p + 1): RK4 stability of DG(p) in 2D
         groups, _ = problems.ensemble(n_instances=64, seed=11)
         return [groups[p] for p in sorted(groups)], \
@@ -67,7 +112,7 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
-def algorithmic_bytes(op, stage, static, k_bytes=8):
+def algorithmic_bytes(op, stage, stat, k_bytes=8):
     """Compulsory HBM bytes of one stage (FP64 state vectors, k vectors of k_bytes per
     value -- 4 on the mixed path, whose k values are FP32 numbers --, int32 topology),
     per kernel.
@@ -75,13 +120,13 @@ def algorithmic_bytes(op, stage, static, k_bytes=8):
     stage (one pass): reads zeta_r, q_r, k_h[1..stage], k_q[1..stage], w_h, w_q; writes
     k_h[stage+1], k_q[stage+1], r_h, r_q (w_h, w_q at the last stage); plus per cell the
     face tables (4 kinds + 8 neighbours + 8 face ids, int32), the cell sizes (2 doubles)
-    and the stored static fields (none for the vortex: every problem-data field is
-    uniform and elided).  Neighbour traces are re-reads of zeta_r, q_r, not compulsory."""
+    and `stat` bytes of stored static fields (static_bytes; none for the vortex: every
+    problem-data field is uniform and elided).  Neighbour traces are re-reads of zeta_r,
+    q_r, not compulsory."""
     n, nd = op.n_cells, op.dofs_per_cell
     v1, v2 = 8 * nd * n, 16 * nd * n
     topo = (4 + 8 + 8) * 4 * n
     geom = 16 * n
-    stat = sum(static.values())
     kf = k_bytes / 8.0
     one_pass = (v1 + v2) * (1 + 2) + (v1 + v2) * kf * (stage + 1) + topo + geom + stat
     rhs = v1 + v2 + v1 + topo + geom
@@ -101,15 +146,29 @@ def cpu_model():
     return "unknown CPU"
 
 
+def dealii_probe():
+    """Whether the reference's own build could run on this host: deal.II's CMake package
+    in the usual install prefixes (the reference needs deal.II >= 9.5 with p4est and MPI)."""
+    import glob
+    pats = ["/usr/lib/cmake/deal.II*/deal.IIConfig.cmake",
+            "/usr/local/lib/cmake/deal.II*/deal.IIConfig.cmake",
+            "/usr/lib/x86_64-linux-gnu/cmake/deal.II*/deal.IIConfig.cmake",
+            "/opt/*/lib/cmake/deal.II*/deal.IIConfig.cmake",
+            "/usr/share/cmake*/deal.II*/deal.IIConfig.cmake"]
+    found = [f for p in pats for f in glob.glob(p)]
+    return {"dealii_found": found, "searched": pats}
+
+
 def cpu_baseline(problem, n_stages, threads=None, min_seconds=1.5, max_stages=256):
-    """The CPU restatement (oracle/) with its loops on `threads` OpenMP threads (default:
-    the box's 16-core share, or fewer if the host has fewer), RK4 stages from the projected
-    initial state, at least n_stages and until min_seconds have passed."""
+    """The CPU restatement (oracle/, its -O3 / FMA / AVX2 build) with its loops on `threads`
+    OpenMP threads (default: the box's 16-core share, or fewer if the host has fewer), RK4
+    stages from the projected initial state, at least n_stages and until min_seconds have
+    passed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from oceano_amd.integrator import butcher_rows
     threads = threads or max(1, min(16, os.cpu_count() or 1))
-    o = pyoracle.Oracle.from_mesh(problem.mesh, problem.degree, bcs=problem.bcs)
+    o = pyoracle.Oracle.from_mesh(problem.mesh, problem.degree, bcs=problem.bcs, fast=True)
     d = o.initialize_data_at_quadrature(problem.problem_data)
     h, q = o.project_hydro(problem.initial, d["cell_2"][:, 0])
     o.set_threads(threads)
@@ -132,8 +191,10 @@ def cpu_baseline(problem, n_stages, threads=None, min_seconds=1.5, max_stages=25
             "kind": "port",
             "sample": "%d RK4 stages of the same workload (%d cells, %d DoF) from the "
                       "projected initial state, %.2f s wall on %d OpenMP threads (%s); "
-                      "the restatement's loops in parallel, face sums atomic"
-                      % (k, problem.mesh.n_cells, dofs, dt, threads, cpu_model())}
+                      "the restatement built -O3 -march=x86-64-v3 with FMA, its loops in "
+                      "parallel, face sums atomic"
+                      % (k, problem.mesh.n_cells, dofs, dt, threads, cpu_model()),
+            "reference_build": dealii_probe()}
 
 
 def main():
@@ -141,9 +202,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed RK stages")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT,
-                    choices=[DEFAULT, "vortex_p1_160x80", "vortex_p1_819k",
-                             "channel_p2_250x10", "ensemble_64"])
+    ap.add_argument("--workload", default=DEFAULT, choices=WORKLOADS)
     ap.add_argument("--dt", type=float, default=None,
                     help="RK4 step (default: the workload's; 2e-5 on the default mesh, "
                          "where 6.5e-5 is unstable)")
@@ -198,12 +257,14 @@ def main():
                     {"options": chosen, "us_per_step": {json.dumps(o): t * 1e6
                                                         for o, t in table}})
             op, data, w_h, w_q = problems.setup_operator(P, precision=precision)
+            stat = static_bytes(op, data)
             if tune_on:
                 tune.apply_options(tune.LIBRARY_DEFAULTS)
             if isinstance(dt, tuple):  # main.cc:1503-1505: dt = CFL / transport speed
                 dt = dt[1] / float("%.3g" % op.transport_speed(w_h, w_q))
             rows, _ = butcher_rows("stage_4_order_4", dt)
             G.append({"P": P, "op": op, "w_h": w_h, "w_q": w_q, "dt": dt, "rows": rows,
+                      "static_bytes": stat,
                       "rk": DeviceIntegrator(op, "stage_4_order_4"),
                       "ki_h": [op.initialize_vector(0) for _ in range(S + 1)],
                       "ki_q": [op.initialize_vector(1) for _ in range(S + 1)],
@@ -228,33 +289,46 @@ def main():
             for _ in range(max(1, -(-args.warmup // S))):
                 g["rk"].step(g["dt"], g["w_h"], g["w_q"])
                 g["steps"] += 1
-        # headline: the device-resident loop -- one captured RK step per operator replayed,
-        # plus any remaining stages eagerly, exactly n_steps_timed stages between two
-        # synchronisations
+        # headline: the device-resident loop -- the timed steps captured into one HIP graph
+        # per operator and replayed once, plus any remaining stages eagerly: exactly
+        # n_steps_timed stages between two synchronisations, timed by HIP events on the
+        # replay stream (the host clock around the region is reported beside it)
         side = torch.cuda.Stream()
+        full, rest = divmod(n_steps_timed, S)
         torch.cuda.synchronize()
         with torch.cuda.stream(side):
             for g in G:
-                g["rk"].capture(g["dt"], g["w_h"], g["w_q"])
-                g["rk"].replay(1)
-                g["steps"] += 1
+                g["rk"].capture(g["dt"], g["w_h"], g["w_q"], n_steps=max(1, full))
+                g["rk"].replay(1)  # first launch of the graph: outside the timed region
+                g["steps"] += max(1, full)
         side.synchronize()
-        full, rest = divmod(n_steps_timed, S)
         # independent operators (the ensemble's degree groups) replay on streams of their
         # own, so that one group's kernels overlap another's tail
         streams = [side] + [torch.cuda.Stream() for _ in G[1:]]
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
         quiesce()
         t0 = time.perf_counter()
+        ev0.record(side)
+        for st in streams[1:]:
+            st.wait_event(ev0)
         for g, st in zip(G, streams):
-            with torch.cuda.stream(st):
-                g["rk"].replay(full)
+            if full:
+                with torch.cuda.stream(st):
+                    g["rk"].replay(1)
         with torch.cuda.stream(side):
             for k in range(rest):
                 stage(G[0], k)
+        for st in streams[1:]:
+            e = torch.cuda.Event()
+            e.record(st)
+            side.wait_event(e)
+        ev1.record(side)
         for st in streams:
             st.synchronize()
         torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
+        wall = time.perf_counter() - t0
+        elapsed = ev0.elapsed_time(ev1) * 1e-3
         gc.enable()
         for g in G:
             g["steps"] += full
@@ -278,7 +352,8 @@ def main():
             rep = op.timing_report()
             op.set_timing(False)
             finish_step(g, n_steps_timed)
-            per_stage = [algorithmic_bytes(op, s, {}, 4 if precision == "mixed" else 8)
+            per_stage = [algorithmic_bytes(op, s, g["static_bytes"],
+                                           4 if precision == "mixed" else 8)
                          for s in range(S)]
             for name, (ms, n) in rep.items():
                 if n == 0:
@@ -295,7 +370,8 @@ def main():
             b = e["bytes"] / n_steps_timed
             kernels[name] = {"mean_ms": mean_ms, "launches": e["launches"], "alg_bytes": b,
                              "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
-        out = {"elapsed": elapsed, "eager": eager, "kernels": kernels, "groups": G,
+        out = {"elapsed": elapsed, "wall": wall, "eager": eager, "kernels": kernels,
+               "groups": G,
                "nonfinite": sum(g["op"].count_nonfinite(g["w_h"]) +
                                 g["op"].count_nonfinite(g["w_q"]) for g in G),
                "sweeps_last_stage": sum(g["op"].last_sweeps() for g in G)}
@@ -328,7 +404,8 @@ def main():
     value = dofs * n_steps_timed / head["elapsed"]
     dom = max(head["kernels"], key=lambda k: head["kernels"][k]["mean_ms"])
     achieved = head["kernels"][dom]["alg_GBs"]
-    traffic = MEASURED_TRAFFIC.get(args.workload, {}).get(head_p) if dom == "stage" else None
+    traffic_tab, traffic_src = measured_traffic(args.workload)
+    traffic = traffic_tab.get(head_p) if dom == "stage" else None
     line = {
         "metric": "DoF-RK-stages/s on 1 GPU, mixed vs FP64 at matched L2 error; % HBM peak",
         "value": value,
@@ -352,11 +429,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "traffic_source": traffic_src if traffic is not None else None,
+                     "alg_bytes_per_launch": head["kernels"][dom]["alg_bytes"]},
+        "timing": "HIP events on the replay stream around the timed region (one captured "
+                  "graph of the timed steps per operator); host clock beside it",
     }
     for p, r in runs.items():
         line[p] = {"value": dofs * n_steps_timed / r["elapsed"],
                    "ms_per_step": r["elapsed"] / n_steps_timed * 1e3,
+                   "wall_value": dofs * n_steps_timed / r["wall"],
                    "eager_value": dofs * n_steps_timed / r["eager"], "kernels": r["kernels"],
                    "nonfinite": r["nonfinite"],
                    "jacobi_sweeps_last_stage": r["sweeps_last_stage"]}
@@ -373,13 +455,19 @@ def main():
             return np.concatenate([s[1].reshape(-1, 2, nd)[:, c].ravel()
                                    for s, nd in zip(states, nds)])
         qn = float(np.sqrt(np.linalg.norm(comp(fs, 0)) ** 2 + np.linalg.norm(comp(fs, 1)) ** 2))
+        zf = np.concatenate([s[0] for s in fs])
+        zm = np.concatenate([s[0] for s in ms])
+        qmax = max(np.abs(comp(fs, 0)).max(), np.abs(comp(fs, 1)).max())
         line["mixed_vs_fp64"] = {
             "speedup": line["mixed"]["value"] / line["fp64"]["value"],
-            "rel_l2_zeta": rel(np.concatenate([s[0] for s in ms]),
-                               np.concatenate([s[0] for s in fs])),
+            "rel_l2_zeta": rel(zm, zf),
             # per component, relative to the norm of the whole discharge (hv ~ 0 in a channel)
             "rel_l2_hu": float(np.linalg.norm(comp(ms, 0) - comp(fs, 0)) / qn),
-            "rel_l2_hv": float(np.linalg.norm(comp(ms, 1) - comp(fs, 1)) / qn)}
+            "rel_l2_hv": float(np.linalg.norm(comp(ms, 1) - comp(fs, 1)) / qn),
+            # nodal maxima, relative to the largest |zeta| and the largest |hu|, |hv|
+            "rel_linf_zeta": float(np.abs(zm - zf).max() / np.abs(zf).max()),
+            "rel_linf_hu": float(np.abs(comp(ms, 0) - comp(fs, 0)).max() / qmax),
+            "rel_linf_hv": float(np.abs(comp(ms, 1) - comp(fs, 1)).max() / qmax)}
     if args.ark and not multi:
         line["ark"] = {p: run_ark(groups[0], dts[0], p, n_steps_timed) for p in precisions}
     if args.steady and not multi:

@@ -210,6 +210,26 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
                               uint32_t n_ki, double *w_h, double *w_q, double *r_h,
                               double *r_q, int32_t write_scratch, void *stream);
 
+/* oc_dg_perform_stage_hydro for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments, results and errors as oc_dg_perform_stage_hydro; it calls no
+ * allocating routine.  The stage kernels' per-batch scratch (Jacobi sweep counts, the
+ * mixed path's deferred depth-solve masks) lives in one static device array per GPU, sized
+ * for OC_NOALLOC_MAX_BATCHES batches of OC_CELL_BATCH cells; an operator with more
+ * batches gets OC_ERR_ARGUMENT.  Every call on a GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads, other operators) race.  Every entry of the
+ * array a stage reads is written earlier in the same stage, so nothing needs clearing.
+ * oc_dg_last_sweeps does not see these calls (their counts stay in the shared array), and
+ * per-kernel timing (oc_dg_set_timing) records nothing for them. */
+#define OC_NOALLOC_MAX_BATCHES 16384
+int oc_dg_perform_stage_hydro_noalloc(oc_dg *op, uint32_t stage, double time,
+                                      const double *factor, uint32_t n_factor,
+                                      const double *ri_h, const double *ri_q,
+                                      const double *ri_vel, double *const *ki_h,
+                                      double *const *ki_q, uint32_t n_ki, double *w_h,
+                                      double *w_q, double *r_h, double *r_q,
+                                      int32_t write_scratch, void *stream);
+
 /* One additive (IMEX) Runge-Kutta stage: the stiff Manning friction is implicit, the rest
  * of the operator explicit.  Replaces the TIMEINTEGRATOR_ADDITIVERUNGEKUTTA overload of
  * OceanoOperator::perform_stage_hydro (OceanDG.h:196-208, body 1936-2119) as called by
@@ -329,7 +349,8 @@ int oc_dg_sum(oc_dg *op, const double *x, int32_t n_components, double *sum, voi
  * time t (oc_rk_set_time, initially 0): stage s of a step runs at t + c_s dt with c_s the
  * row sum of a[s] (ExplicitRungeKuttaIntegrator.h:212-271), and every executed step
  * advances t by dt.  oc_rk_capture records one step with fixed arguments into a HIP graph
- * (on a non-null stream) without running it; oc_rk_replay launches it n times.  With
+ * (on a non-null stream) without running it; oc_rk_replay launches it n times (n times
+ * the recorded steps).  With
  * time-dependent boundary data (oc_dg_set_boundary_table) the recorded stages read the
  * table through a device-side entry counter that oc_rk_replay sets from t and the graph
  * advances; the replay checks on the host that the table lists the n steps' stage times.
@@ -345,6 +366,10 @@ int oc_rk_time(const oc_rk *rk, double *t);
 int oc_rk_step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity, void *stream);
 int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity,
                   void *stream);
+/* n_steps consecutive steps in one graph (one launch per replay runs them all: the host
+ * launch and the gap between graph launches are paid once per n_steps steps) */
+int oc_rk_capture_steps(oc_rk *rk, int32_t n_steps, double dt, double *w_h, double *w_q,
+                        double *velocity, void *stream);
 int oc_rk_replay(oc_rk *rk, int32_t n, void *stream);
 /* device pointers of the integrator's registers (arrays of S+1 and 2 entries; any may be
  * null) */

@@ -8,9 +8,25 @@
 // into exceptions, which is the reference's error convention (main.cc:1709-1734).
 //
 // VectorType must provide `double *get_values()` / `const double *get_values() const`
-// returning device memory of `size()` doubles in the operator's storage order, and
-// `reinit(size_t n)`; oceano::DeviceVector below does, and so does deal.II's
-// LinearAlgebra::distributed::Vector<double, MemorySpace::Default>.
+// returning device memory of `size()` doubles, and `reinit(size_t n)`; oceano::DeviceVector
+// below does, and so does deal.II's LinearAlgebra::distributed::Vector<double,
+// MemorySpace::Default>.
+//
+// Vector layout (the Layout argument of the constructor):
+//   operator_order  the caller's vectors are in the operator's storage order (what
+//                   initialize_vector sizes them for, as MatrixFree::initialize_dof_vector
+//                   does); every call passes them straight to the kernels.
+//   cell_major      the caller's vectors are in deal.II's DoF order for a DG FESystem: cell
+//                   by cell in the order of the mesh given at construction, component-major
+//                   inside a cell, lexicographic Gauss-Lobatto nodes (FE_DGQ numbering;
+//                   main.cc:580-583).  The adapter keeps operator-order mirrors of the time
+//                   integrator's registers: stage 0 packs the solution into its mirror, the
+//                   stages run on the mirrors, the last stage unpacks the new solution back
+//                   into the caller's vectors (2 permutations of the state per step).  The
+//                   caller's next_ri / vec_ki vectors are then scratch the operator does not
+//                   read back; only the solution carries the state between steps, which is
+//                   all the reference time loop reads (output, errors, mass check).
+//                   apply() packs its inputs and unpacks its outputs on every call.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -107,14 +123,25 @@ private:
   size_t n_ = 0;
 };
 
+enum class Layout
+{
+  operator_order,
+  cell_major
+};
+
 template <typename VectorType>
 class OceanoOperator
 {
 public:
-  OceanoOperator(const oc_mesh_desc &mesh, const oc_params &params, hipStream_t stream = nullptr)
-    : stream_(stream)
+  OceanoOperator(const oc_mesh_desc &mesh, const oc_params &params, hipStream_t stream = nullptr,
+                 Layout layout = Layout::operator_order)
+    : stream_(stream), layout_(layout)
   {
     check(oc_dg_create(&mesh, &params, &op_), "oc_dg_create");
+    int64_t sizes[7];
+    check(oc_dg_sizes(op_, sizes), "oc_dg_sizes");
+    n_cells_ = sizes[0];
+    dofs_per_cell_ = sizes[2];
   }
   OceanoOperator(const OceanoOperator &) = delete;
   OceanoOperator &operator=(const OceanoOperator &) = delete;
@@ -125,13 +152,12 @@ public:
   }
 
   oc_dg *handle() const { return op_; }
+  Layout layout() const { return layout_; }
 
   // OceanDG.h:346-348: variable 0 = free surface, 1 = discharge (and velocity)
   void initialize_vector(VectorType &vector, const unsigned int variable) const
   {
-    int64_t n = 0;
-    check(oc_dg_vector_length(op_, variable == 0 ? 1 : 2, &n), "oc_dg_vector_length");
-    vector.reinit((size_t)n);
+    vector.reinit(length(variable == 0 ? 1 : 2, layout_));
   }
 
   // OceanDG.h:350 + 498-569: problem_data(x, y, out[5]) evaluated at the quadrature points
@@ -178,13 +204,44 @@ public:
     const double *vel = current_ri.size() > 3 && current_ri.back().size()
                           ? current_ri.back().get_values()
                           : nullptr;
+    if (layout_ == Layout::operator_order)
+      {
+        check(oc_dg_perform_stage_hydro(op_, cur_stage, cur_time, factor_residual,
+                                        cur_stage + 1, current_ri[0].get_values(),
+                                        current_ri[1].get_values(), vel, kh.data(), kq.data(),
+                                        (uint32_t)n_ki, solution_height.get_values(),
+                                        solution_discharge.get_values(),
+                                        next_ri_height.get_values(),
+                                        next_ri_discharge.get_values(), 0, stream_),
+              "oc_dg_perform_stage_hydro");
+        return;
+      }
+    // cell_major: the stages run on operator-order mirrors of the integrator's registers
+    Mirrors &m = mirrors(n_ki, n_ki);
+    if (cur_stage == 0)
+      {
+        to_operator(1, solution_height.get_values(), m.w_h.get_values());
+        to_operator(2, solution_discharge.get_values(), m.w_q.get_values());
+      }
+    if (vel)
+      {
+        to_operator(2, vel, m.vel.get_values());
+        vel = m.vel.get_values();
+      }
+    const bool last = cur_stage + 2 == n_ki;
+    const double *ri_h = cur_stage == 0 ? m.w_h.get_values() : m.r_h[cur_stage % 2].get_values();
+    const double *ri_q = cur_stage == 0 ? m.w_q.get_values() : m.r_q[cur_stage % 2].get_values();
     check(oc_dg_perform_stage_hydro(op_, cur_stage, cur_time, factor_residual, cur_stage + 1,
-                                    current_ri[0].get_values(), current_ri[1].get_values(), vel,
-                                    kh.data(), kq.data(), (uint32_t)n_ki,
-                                    solution_height.get_values(),
-                                    solution_discharge.get_values(), next_ri_height.get_values(),
-                                    next_ri_discharge.get_values(), 0, stream_),
+                                    ri_h, ri_q, vel, m.kh_ptr.data(), m.kq_ptr.data(),
+                                    (uint32_t)n_ki, m.w_h.get_values(), m.w_q.get_values(),
+                                    m.r_h[(cur_stage + 1) % 2].get_values(),
+                                    m.r_q[(cur_stage + 1) % 2].get_values(), 0, stream_),
           "oc_dg_perform_stage_hydro");
+    if (last)
+      {
+        to_cell_major(1, m.w_h.get_values(), solution_height.get_values());
+        to_cell_major(2, m.w_q.get_values(), solution_discharge.get_values());
+      }
   }
 
   // OceanDG.h:196-208: the TIMEINTEGRATOR_ADDITIVERUNGEKUTTA overload; reads the public
@@ -208,32 +265,154 @@ public:
     const double *vel = current_ri.size() > 3 && current_ri.back().size()
                           ? current_ri.back().get_values()
                           : nullptr;
+    if (layout_ == Layout::operator_order)
+      {
+        check(oc_dg_perform_stage_hydro_ark(
+                op_, cur_stage, cur_time, factor_residual, factor_tilde_residual,
+                cur_stage + 1, factor_matrix, current_ri[0].get_values(),
+                current_ri[1].get_values(), vel, kh.data(), (uint32_t)n_h, kq.data(),
+                (uint32_t)n_q, solution_height.get_values(), solution_discharge.get_values(),
+                next_ri_height.get_values(), next_ri_discharge.get_values(), stream_),
+              "oc_dg_perform_stage_hydro_ark");
+        return;
+      }
+    Mirrors &m = mirrors(n_h, n_q);
+    if (cur_stage == 0)
+      {
+        to_operator(1, solution_height.get_values(), m.w_h.get_values());
+        to_operator(2, solution_discharge.get_values(), m.w_q.get_values());
+      }
+    if (vel)
+      {
+        to_operator(2, vel, m.vel.get_values());
+        vel = m.vel.get_values();
+      }
+    const bool last = cur_stage + 2 == n_h;
+    const double *ri_h = cur_stage == 0 ? m.w_h.get_values() : m.r_h[cur_stage % 2].get_values();
+    const double *ri_q = cur_stage == 0 ? m.w_q.get_values() : m.r_q[cur_stage % 2].get_values();
     check(oc_dg_perform_stage_hydro_ark(
             op_, cur_stage, cur_time, factor_residual, factor_tilde_residual, cur_stage + 1,
-            factor_matrix, current_ri[0].get_values(), current_ri[1].get_values(), vel,
-            kh.data(), (uint32_t)n_h, kq.data(), (uint32_t)n_q, solution_height.get_values(),
-            solution_discharge.get_values(), next_ri_height.get_values(),
-            next_ri_discharge.get_values(), stream_),
+            factor_matrix, ri_h, ri_q, vel, m.kh_ptr.data(), (uint32_t)n_h, m.kq_ptr.data(),
+            (uint32_t)n_q, m.w_h.get_values(), m.w_q.get_values(),
+            m.r_h[(cur_stage + 1) % 2].get_values(), m.r_q[(cur_stage + 1) % 2].get_values(),
+            stream_),
           "oc_dg_perform_stage_hydro_ark");
+    if (last)
+      {
+        to_cell_major(1, m.w_h.get_values(), solution_height.get_values());
+        to_cell_major(2, m.w_q.get_values(), solution_discharge.get_values());
+      }
   }
 
   // d~_stage dt of the implicit stage, set by AdditiveRungeKuttaIntegrator (OceanDG.h:163)
   double factor_matrix = 0.0;
 
-  // OceanDG.h:178-180 with dst = M^-1 L(src) for both fields
+  // OceanDG.h:178-180, apply(current_time, src, dst): dst = M^-1 L(src) for both fields
+  // (step-67 semantics; the reference's body is dead code), with the boundary data at
+  // current_time.  The fields travel as the stage's current_ri does: src = {free surface,
+  // discharge[, tracer[, velocity]]} (the velocity is read when the viscosity is nonzero),
+  // dst = {free surface, discharge}.
+  void apply(const double current_time, const std::vector<VectorType> &src,
+             std::vector<VectorType> &dst) const
+  {
+    if (src.size() < 2 || dst.size() < 2)
+      throw Error(OC_ERR_ARGUMENT, "apply: src needs {zeta, q, ...}, dst {zeta, q}");
+    const VectorType *vel = src.size() > 3 && src.back().size() ? &src.back() : nullptr;
+    apply(current_time, src[0], src[1], dst[0], dst[1], vel);
+  }
+
+  // the same with the fields as separate arguments
   void apply(const double current_time, const VectorType &src_height,
              const VectorType &src_discharge, VectorType &dst_height,
              VectorType &dst_discharge, const VectorType *velocity = nullptr) const
   {
-    check(oc_dg_apply(op_, current_time, src_height.get_values(), src_discharge.get_values(),
-                      velocity ? velocity->get_values() : nullptr, dst_height.get_values(),
-                      dst_discharge.get_values(), stream_),
+    if (layout_ == Layout::operator_order)
+      {
+        check(oc_dg_apply(op_, current_time, src_height.get_values(),
+                          src_discharge.get_values(),
+                          velocity ? velocity->get_values() : nullptr, dst_height.get_values(),
+                          dst_discharge.get_values(), stream_),
+              "oc_dg_apply");
+        return;
+      }
+    Mirrors &m = mirrors(2, 2);
+    to_operator(1, src_height.get_values(), m.a_h.get_values());
+    to_operator(2, src_discharge.get_values(), m.a_q.get_values());
+    if (velocity)
+      to_operator(2, velocity->get_values(), m.vel.get_values());
+    check(oc_dg_apply(op_, current_time, m.a_h.get_values(), m.a_q.get_values(),
+                      velocity ? m.vel.get_values() : nullptr, m.b_h.get_values(),
+                      m.b_q.get_values(), stream_),
           "oc_dg_apply");
+    to_cell_major(1, m.b_h.get_values(), dst_height.get_values());
+    to_cell_major(2, m.b_q.get_values(), dst_discharge.get_values());
   }
 
 private:
+  // operator-order registers of the cell_major layout
+  struct Mirrors
+  {
+    DeviceVector w_h, w_q, vel, r_h[2], r_q[2], a_h, a_q, b_h, b_q;
+    std::vector<DeviceVector> kh, kq;
+    std::vector<double *> kh_ptr, kq_ptr;
+  };
+
+  size_t length(int n_components, Layout layout) const
+  {
+    if (layout == Layout::cell_major)
+      return (size_t)n_cells_ * n_components * dofs_per_cell_;
+    int64_t n = 0;
+    check(oc_dg_vector_length(op_, n_components, &n), "oc_dg_vector_length");
+    return (size_t)n;
+  }
+
+  Mirrors &mirrors(size_t n_kh, size_t n_kq) const
+  {
+    Mirrors &m = mirrors_;
+    if (m.w_h.size() == 0)
+      {
+        const size_t n1 = length(1, Layout::operator_order);
+        const size_t n2 = length(2, Layout::operator_order);
+        m.w_h.reinit(n1);
+        m.w_q.reinit(n2);
+        m.vel.reinit(n2);
+        m.a_h.reinit(n1);
+        m.a_q.reinit(n2);
+        m.b_h.reinit(n1);
+        m.b_q.reinit(n2);
+        for (int i = 0; i < 2; ++i)
+          {
+            m.r_h[i].reinit(n1);
+            m.r_q[i].reinit(n2);
+          }
+      }
+    while (m.kh.size() < n_kh)
+      m.kh.emplace_back(length(1, Layout::operator_order));
+    while (m.kq.size() < n_kq)
+      m.kq.emplace_back(length(2, Layout::operator_order));
+    m.kh_ptr.resize(m.kh.size());
+    m.kq_ptr.resize(m.kq.size());
+    for (size_t j = 0; j < m.kh.size(); ++j)
+      m.kh_ptr[j] = m.kh[j].get_values();
+    for (size_t j = 0; j < m.kq.size(); ++j)
+      m.kq_ptr[j] = m.kq[j].get_values();
+    return m;
+  }
+
+  void to_operator(int n_components, const double *cell_major, double *op_order) const
+  {
+    check(oc_dg_pack(op_, n_components, cell_major, op_order, stream_), "oc_dg_pack");
+  }
+  void to_cell_major(int n_components, const double *op_order, double *cell_major) const
+  {
+    check(oc_dg_unpack(op_, n_components, op_order, cell_major, stream_), "oc_dg_unpack");
+  }
+
   oc_dg *op_ = nullptr;
   hipStream_t stream_ = nullptr;
+  Layout layout_ = Layout::operator_order;
+  int64_t n_cells_ = 0, dofs_per_cell_ = 0;
+  mutable Mirrors mirrors_;
 };
 
 // TimeIntegrator::ExplicitRungeKuttaIntegrator (ExplicitRungeKuttaIntegrator.h:77-298) with

@@ -18,6 +18,7 @@ OC_ERR_BOUNDARY = 3
 OC_ERR_HIP = 4
 OC_ERR_NONFINITE = 5
 OC_ERR_UNSUPPORTED = 6
+OC_NOALLOC_MAX_BATCHES = 16384
 
 OC_BC_WALL = 1
 OC_BC_SUPERCRITICAL_INFLOW = 2
@@ -78,6 +79,8 @@ SIGNATURES = {
     "oc_rk_capture": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
     "oc_rk_replay": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "oc_rk_capture_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     "oc_rk_set_time": (C.c_int, [C.c_void_p, C.c_double]),
     "oc_rk_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "oc_rk_registers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
@@ -102,6 +105,9 @@ SIGNATURES = {
     "oc_dg_unpack": (C.c_int, [P, I32, P, P, P]),
     "oc_dg_perform_stage_hydro": (C.c_int, [P, U32, D, PD, U32, P, P, P, C.POINTER(P),
                                             C.POINTER(P), U32, P, P, P, P, I32, P]),
+    "oc_dg_perform_stage_hydro_noalloc": (C.c_int, [P, U32, D, PD, U32, P, P, P,
+                                                    C.POINTER(P), C.POINTER(P), U32, P, P, P,
+                                                    P, I32, P]),
     "oc_dg_perform_stage_hydro_ark": (C.c_int, [P, U32, D, PD, PD, U32, D, P, P, P,
                                                 C.POINTER(P), U32, C.POINTER(P), U32, P, P,
                                                 P, P, P]),

@@ -13,8 +13,12 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liboceano_dg.so")
 
-SOURCES = ["mesh.cpp", "icbc.cpp", "dg_operator.hip", "dg_rk.hip", "mlp.hip",
-           "dg_stage_p1.hip", "dg_stage_p2.hip", "dg_stage_p3.hip", "dg_stage_p4.hip"]
+# (source, extra flags, object name): the per-degree stage source is compiled once per degree
+SOURCES = [("mesh.cpp", [], "mesh"), ("icbc.cpp", [], "icbc"),
+           ("dg_operator.hip", [], "dg_operator"), ("dg_rk.hip", [], "dg_rk"),
+           ("mlp.hip", [], "mlp")] + \
+          [("dg_stage_degree.hip", ["-DOC_DEGREE=%d" % p], "dg_stage_p%d" % p)
+           for p in (1, 2, 3, 4)]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -23,6 +27,10 @@ COMMON = [
     "--offload-arch=gfx950",
     # packed FP32 ops are disabled for every HIP translation unit
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 division and square root as the hardware's v_rcp_f32 / v_sqrt_f32 (1 ulp)
+    # instead of the correctly rounded multi-instruction sequences: only the FP32 arithmetic
+    # of the mixed-precision path and the closure network is affected (FP64 code is not)
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-I" + os.path.join(ROOT, "include"),
     "-I" + CSRC,
     "-Wno-unused-command-line-argument",
@@ -42,14 +50,14 @@ def build(verbose=False, extra=(), incremental=False):
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
     procs = []
     newest_header = _headers()
-    for src in SOURCES:
+    for src, flags, name in SOURCES:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(HERE, "_build", src + ".o")
+        obj = os.path.join(HERE, "_build", name + ".o")
         if (incremental and os.path.exists(obj)
                 and os.path.getmtime(obj) > max(os.path.getmtime(path), newest_header)):
             objs.append(obj)
             continue
-        cmd = [HIPCC, *COMMON, *extra, "-x", "hip", "-c", path, "-o", obj]
+        cmd = [HIPCC, *COMMON, *flags, *extra, "-x", "hip", "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd)))

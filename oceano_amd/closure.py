@@ -416,26 +416,33 @@ def _frozen_source(op, data, mesh, tau_nodal):
     op.set_static_data("cell_0", d0)
 
 
-def evaluate(sample, net=None, cfl=None, exact=True):
+def evaluate(sample, net=None, cfl=None, exact=True, gains=(), baselines=True):
     """coarse runs of one held-out sample from the restricted fine initial state to the
-    final time: plain sub-grid model, plain + network, the pointwise-sampled bed and (the
-    ceiling of any momentum-source correction) plain + the exact correction of the initial
-    state frozen in time; returns the L2 errors (zeta, hu, hv) of each against the
-    restricted fine solution"""
+    final time: plain sub-grid model, plain + network (at the network's own output gain,
+    and as "learned@g" at each gain g of `gains`), the pointwise-sampled bed (the
+    reference's own sampling of the bed, OceanDG.h:498-569) and (the ceiling of any
+    momentum-source correction) plain + the exact correction of the initial state frozen in
+    time; returns the L2 errors (zeta, hu, hv) of each against the restricted fine
+    solution"""
     S = sample.S
     t0, rh0, rq0 = sample.states[0]
     tT, rhT, rqT = sample.states[-1]
     cfl = cfl or sample.cfl
     out = {}
-    cases = [("plain", S.plain, False), ("learned", S.plain, True),
-             ("sampled", problems.channel(degree=S.degree, nx=S.nx, ny=S.ny, seed=S.seed,
-                                          manning=S.manning), False),
-             ("exact", S.plain, False)]
+    cases = [("plain", S.plain, False, None), ("learned", S.plain, True, None)]
+    cases += [("learned@%g" % g, S.plain, True, g) for g in gains]
+    if baselines:
+        cases += [("sampled", problems.channel(degree=S.degree, nx=S.nx, ny=S.ny, seed=S.seed,
+                                               manning=S.manning), False, None),
+                  ("exact", S.plain, False, None)]
     hx, hy = S.length / S.nx, 1.0 / S.ny
     nd = (S.degree + 1) ** 2
-    for name, prob, with_net in cases:
+    own_gain = net.output_gain if net is not None else 1.0
+    for name, prob, with_net, gain in cases:
         if (with_net and net is None) or (name == "exact" and not exact):
             continue
+        if with_net:
+            net.set_output_gain(own_gain if gain is None else gain)
         op, data, _, _ = problems.setup_operator(prob)
         if name == "exact":
             _, V = nodal_to_modal(S.degree)
@@ -453,15 +460,45 @@ def evaluate(sample, net=None, cfl=None, exact=True):
         out[name] = [eh[0], eq[0], eq[1]]
         if with_net:
             op.set_closure(None)
+    if net is not None:
+        net.set_output_gain(own_gain)
     return out
 
 
+GAINS = (0.0, 0.03125, 0.0625, 0.125, 0.25, 0.5, 1.0)
+
+
+def calibrate_gain(net, seeds, gains=GAINS, T=2.0, n_snap=2, log=None):
+    """A-posteriori calibration of the correction on training bathymetries: coupled coarse
+    runs with the network's output scaled by each gain, scored by the sum over the runs of
+    the zeta and hu errors relative to the plain model's; the best gain is kept (0, the
+    plain model, when every nonzero gain degrades the coupled runs).  Returns (gain,
+    {gain: score})."""
+    score = {g: 0.0 for g in gains}
+    for seed in seeds:
+        s = Sample(seed, T=T, n_snap=n_snap)
+        r = evaluate(s, net, gains=gains, baselines=False)
+        pz, pq = r["plain"][0], r["plain"][1]
+        for g in gains:
+            e = r["learned@%g" % g]
+            score[g] += e[0] / pz + e[1] / pq
+        if log:
+            log("calibration seed %d: " % seed +
+                " ".join("g=%g z %.2e hu %.2e" % (g, r["learned@%g" % g][0],
+                                                 r["learned@%g" % g][1]) for g in gains))
+        del s
+    best = min(gains, key=lambda g: (score[g], g))
+    return best, score
+
+
 def run_experiment(n_train=256, n_eval=32, steps=20000, noise=0.02, T=2.0, n_snap=4,
-                   train_seed0=1000, eval_seed0=2000, net_seed=0, log=print):
+                   train_seed0=1000, eval_seed0=2000, net_seed=0, n_cal=8, log=print):
     """The closure experiment: fine runs of n_train seeded bathymetries give the training
-    data, the network is trained on the GPU, and n_eval held-out bathymetries are run with
-    the plain sub-grid model, plain + network, the sampled bed and the exact-correction
-    ceiling.  Returns a report dict (errors are root-sum-squares over the held-out runs)."""
+    data, the network is trained on the GPU, its output gain is calibrated on coupled
+    coarse runs of n_cal training bathymetries (calibrate_gain), and n_eval held-out
+    bathymetries are run with the plain sub-grid model, plain + network, the sampled bed and
+    the exact-correction ceiling.  Returns a report dict (errors are root-sum-squares over
+    the held-out runs)."""
     import time
 
     import torch
@@ -483,14 +520,34 @@ def run_experiment(n_train=256, n_eval=32, steps=20000, noise=0.02, T=2.0, n_sna
     if log:
         log("trained on %d samples in %.1f s (%d Adam steps), final loss %.4e" %
             (X.shape[0], t_fit, steps, hist[-1][1] if hist else float("nan")))
+    gain, score = calibrate_gain(net, range(train_seed0, train_seed0 + n_cal), T=T, log=log)
+    net.set_output_gain(gain)
+    rep = {
+        "n_train": n_train, "n_eval": n_eval, "adam_steps": steps, "input_noise": noise,
+        "time_s": T, "snapshots": n_snap + 1, "train_samples": int(X.shape[0]),
+        "train_seconds": t_fit, "calibration_seeds": n_cal, "gain": gain,
+        "calibration_scores": {str(k): v for k, v in score.items()}}
+    rep.update(evaluate_heldout(net, n_eval, T, n_snap, eval_seed0, log=log))
+    rep["total_seconds"] = time.time() - t0
+    return rep, net
+
+
+def evaluate_heldout(net, n_eval=32, T=2.0, n_snap=4, eval_seed0=2000, log=None):
+    """root-sum-square errors over n_eval held-out bathymetries: plain, plain + network at
+    its calibrated gain ("learned") and at gain 1 ("learned@1"), the sampled bed and the
+    exact-correction ceiling; with the gains against the plain model and the sampled bed"""
+    import torch
     scale = torch.tensor(net.normalization[3], device="cuda", dtype=torch.float32)
     tot, apriori = {}, []
     for i in range(n_eval):
         ev = Sample(eval_seed0 + i, T=T, n_snap=n_snap)
+        g = net.output_gain
+        net.set_output_gain(1.0)
         loss, _ = net.gradients(ev.X, ev.T)
+        net.set_output_gain(g)
         zero = float(((ev.T / scale) ** 2).mean())  # loss of no correction
         apriori.append((loss, zero))
-        r = evaluate(ev, net)
+        r = evaluate(ev, net, gains=(1.0,) if g != 1.0 else ())
         for k, v in r.items():
             tot.setdefault(k, np.zeros(3))
             tot[k] += np.array(v) ** 2
@@ -500,17 +557,14 @@ def run_experiment(n_train=256, n_eval=32, steps=20000, noise=0.02, T=2.0, n_sna
                 " ".join("%s z %.2e hu %.2e" % (k, v[0], v[1]) for k, v in r.items()))
         del ev
     err = {k: [float(x) for x in np.sqrt(v)] for k, v in tot.items()}
-    rep = {
-        "n_train": n_train, "n_eval": n_eval, "adam_steps": steps, "input_noise": noise,
-        "time_s": T, "snapshots": n_snap + 1, "train_samples": int(X.shape[0]),
-        "train_seconds": t_fit, "total_seconds": time.time() - t0,
+    return {
         "apriori_loss_network": float(np.mean([a for a, _ in apriori])),
         "apriori_loss_no_correction": float(np.mean([b for _, b in apriori])),
         "errors_zeta_hu_hv": err,
         "gain_vs_plain": {k: [err["plain"][j] / v[j] for j in range(2)]
                           for k, v in err.items()},
-    }
-    return rep, net
+        "gain_vs_sampled_bed": {k: [err["sampled"][j] / v[j] for j in range(2)]
+                                for k, v in err.items()} if "sampled" in err else None}
 
 
 def main(argv=None):
@@ -522,12 +576,34 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--noise", type=float, default=0.02)
     ap.add_argument("--time", type=float, default=2.0)
+    ap.add_argument("--calibrate", default=None,
+                    help="a trained network (.npz): calibrate its output gain on training "
+                         "bathymetries and evaluate it on the held-out ones (no training)")
+    ap.add_argument("--cal-seeds", type=int, default=8)
     ap.add_argument("--out", default="closure_run")
     a = ap.parse_args(argv)
     import os
     os.makedirs(a.out, exist_ok=True)
-    rep, net = run_experiment(a.train, a.eval, a.steps, a.noise, a.time,
-                              log=lambda m: print(m, flush=True))
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    if a.calibrate:
+        import time
+        from .mlp import ClosureNetwork
+        t0 = time.time()
+        net = ClosureNetwork.load(a.calibrate)
+        net.set_output_gain(1.0)
+        gain, score = calibrate_gain(net, range(1000, 1000 + a.cal_seeds), T=a.time, log=log)
+        net.set_output_gain(gain)
+        log("calibrated output gain %g (scores %s)" % (gain, score))
+        rep = {"calibration_seeds": a.cal_seeds, "gain": gain,
+               "calibration_scores": {str(k): v for k, v in score.items()}}
+        rep.update(evaluate_heldout(net, a.eval, a.time, log=log))
+        rep["total_seconds"] = time.time() - t0
+        net.save(os.path.join(a.out, "closure_channel_p2.npz"))
+        with open(os.path.join(a.out, "report.json"), "w") as f:
+            json.dump(rep, f, indent=1)
+        print(json.dumps(rep))
+        return
+    rep, net = run_experiment(a.train, a.eval, a.steps, a.noise, a.time, log=log)
     net.save(os.path.join(a.out, "closure_channel_p2.npz"))
     with open(os.path.join(a.out, "report.json"), "w") as f:
         json.dump(rep, f, indent=1)

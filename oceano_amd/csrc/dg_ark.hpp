@@ -362,7 +362,7 @@ int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stag
   J.stage = (int)stage;
   J.wsol = w_h;
   J.rk_dst = last ? w_h : r_h;
-  J.sweeps_blk = op->d_sweeps_blk;
+  J.sweeps_blk = O.sweeps;
   timed_begin(op, s, &ev);
   hipLaunchKernelGGL((k_height_jacobi<P, R, R>), grid, block, 0, s, O, J);
   timed_end(op, s, TK_HEIGHT_JACOBI, ev);

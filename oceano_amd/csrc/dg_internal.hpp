@@ -19,6 +19,7 @@ void set_error(const std::string &msg);
 struct oc_dg
 {
   using OpArgs = oceano::OpArgs;
+  int device = 0;  // the GPU current at oc_dg_create
   int P, n1, nd, nq, ng;
   int N, NB;
   int precision;
@@ -60,6 +61,7 @@ struct oc_dg
   uint64_t generation = 0;
   unsigned int *d_sweeps_blk = nullptr;
   float *d_jmat = nullptr;  // mixed precision: closed-form Jacobi matrix (unit cell)
+  float *d_kinv = nullptr;  // mixed precision: momentum inverse mass (unit cell)
   unsigned long long *d_defer = nullptr;  // mixed precision: per batch, deferred lanes
   double *d_post = nullptr; // per-batch results of the reductions [NB]
   double wet_margin = 10.0;
@@ -107,8 +109,10 @@ struct oc_dg
     o.max_it = max_it;
     o.xcd_swizzle = xcd_swizzle;
     o.jmat = d_jmat;
+    o.kinv = d_kinv;
     o.wet_margin = wet_margin;
     o.defer = d_defer;
+    o.sweeps = d_sweeps_blk;
     o.closure = nullptr;  // set per stage when a closure network is attached
     return o;
   }

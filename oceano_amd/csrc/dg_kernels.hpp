@@ -84,8 +84,12 @@ struct OpArgs
   // mixed precision: closed form of max_it linear Jacobi sweeps on a wet unit cell,
   // [nd][nd] row-major (delta = J r / area), and the wet-cell safety factor
   const float *jmat;
+  // mixed precision: the momentum inverse mass of the unit cell, [nd][nd] row-major
+  // (M^-1 = K / area on an affine cell)
+  const float *kinv;
   double wet_margin;
   unsigned long long *defer;  // mixed path: per batch, lanes whose depth solve is deferred
+  unsigned int *sweeps;       // per batch: Jacobi sweeps of the last stage (sum over lanes)
   const double *closure;  // [batch][2][nd][lane] M^-1-form momentum source, or nullptr
 };
 

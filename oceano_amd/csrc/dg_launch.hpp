@@ -75,7 +75,7 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
   H.wsol = w_h;
   H.rk_dst = h_dst;
   H.scratch = write_scratch ? ki_h[0] : nullptr;
-  H.sweeps_blk = op->d_sweeps_blk;
+  H.sweeps_blk = O.sweeps;
   // the one-pass stage reads neighbour traces of ri while other cells write the outputs:
   // it needs outputs distinct from the inputs (single-stage schemes write w in place)
   const bool alias = (h_dst == ri_h) || (q_dst == ri_q) || (h_dst == ri_q) || (q_dst == ri_h);

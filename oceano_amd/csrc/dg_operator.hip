@@ -156,6 +156,27 @@ std::vector<float> jacobi_matrix(int p, int K)
   return out;
 }
 
+// The momentum inverse mass of the unit cell as one matrix: the basis change of
+// CellwiseInverseMassMatrix (OceanDG.h:1541-1553) from GL(p+1) nodal values to Gauss(p+1)
+// collocation, the inverse quadrature weights and back, is B1 (x) B1 with
+// B1[i][j] = sum_a Ti[i][a] Ti[j][a] / wn[a]; on a cell of area |K|, M^-1 = (B1 (x) B1) / |K|.
+// Row-major [nd][nd], node index i + n1 j (x fastest).
+std::vector<float> inverse_mass_matrix(int p)
+{
+  const DegreeTables t = make_tables(p);
+  const int n1 = p + 1, nd = n1 * n1;
+  std::vector<double> B(n1 * n1, 0.0);
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j)
+      for (int a = 0; a < n1; ++a)
+        B[i * n1 + j] += t.Ti[i][a] * t.Ti[j][a] / t.wn[a];
+  std::vector<float> out(nd * nd);
+  for (int r = 0; r < nd; ++r)
+    for (int c = 0; c < nd; ++c)
+      out[r * nd + c] = float(B[(r % n1) * n1 + (c % n1)] * B[(r / n1) * n1 + (c / n1)]);
+  return out;
+}
+
 int closure_dispatch(oc_dg *op, const OpArgs &O, const double *zeta, const double *q,
                      float *features, hipStream_t s)
 {
@@ -189,6 +210,25 @@ double legendre01(int n, double t)
 }
 
 bool g_tables_ready[64] = {};
+
+// Per-batch scratch of the stage kernels for oc_dg_perform_stage_hydro_noalloc: deferred
+// depth-solve masks (one 64-bit word per batch), then Jacobi sweep counts (one 32-bit word
+// per batch).  Each GPU holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device (g_scratch).
+constexpr int NOALLOC_WORDS = OC_NOALLOC_MAX_BATCHES + OC_NOALLOC_MAX_BATCHES / 2;
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_WORDS];
+unsigned long long *g_scratch[64] = {};
+
+// once per device, with that device current (called by oc_dg_create)
+int ensure_scratch(int dev)
+{
+  if (g_scratch[dev])
+    return OC_OK;
+  void *ptr = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch[dev] = static_cast<unsigned long long *>(ptr);
+  return OC_OK;
+}
 
 int ensure_tables()
 {
@@ -383,6 +423,7 @@ void free_all(oc_dg *op)
   f(op->d_btable);
   f(op->d_sweeps_blk);
   f(op->d_jmat);
+  f(op->d_kinv);
   f(op->d_defer);
   f(op->d_post);
   f(op->d_count);
@@ -740,7 +781,14 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       hxy[((size_t)(i / CB) * 2 + 0) * CB + i % CB] = op->geom[4 * c + 2];
       hxy[((size_t)(i / CB) * 2 + 1) * CB + i % CB] = op->geom[4 * c + 3];
     }
-  if ((rc = ensure_tables()))
+  if ((rc = hipGetDevice(&op->device) == hipSuccess ? OC_OK : OC_ERR_HIP) ||
+      op->device < 0 || op->device >= 64)
+    {
+      set_error("oc_dg_create: no usable current device");
+      delete op;
+      return OC_ERR_HIP;
+    }
+  if ((rc = ensure_tables()) || (rc = ensure_scratch(op->device)))
     {
       delete op;
       return rc;
@@ -750,6 +798,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       (rc = upload(&op->d_fid, fid)) || (rc = upload(&op->d_finfo, finfo)) ||
       (rc = upload(&op->d_bkind, bkind)) || (rc = dalloc(&op->d_sweeps_blk, (size_t)NB)) ||
       (rc = upload(&op->d_jmat, jacobi_matrix(op->P, op->max_it))) ||
+      (rc = upload(&op->d_kinv, inverse_mass_matrix(op->P))) ||
       (rc = dalloc(&op->d_defer, (size_t)NB)) || (rc = dalloc(&op->d_post, (size_t)NB)) ||
       (rc = dalloc(&op->d_count, 1)))
     {
@@ -1055,8 +1104,11 @@ int oc_dg_set_boundary_table(oc_dg *op, int32_t n_times, const double *times,
 {
   if (!op || n_times < 1 || !times)
     return bad_argument(__func__);
+  // non-decreasing up to the matching tolerance: stage times computed as t0 + r dt + c dt
+  // may step back by a rounding error where one step's last stage meets the next's first
   for (int32_t k = 0; k < n_times; ++k)
-    if (!std::isfinite(times[k]) || (k > 0 && times[k] < times[k - 1]))
+    if (!std::isfinite(times[k]) ||
+        (k > 0 && times[k] < times[k - 1] - 1e-12 * std::max(1.0, std::fabs(times[k]))))
       {
         set_error("oc_dg_set_boundary_table: times must be finite and non-decreasing");
         return OC_ERR_ARGUMENT;
@@ -1098,11 +1150,14 @@ int oc_dg_unpack(oc_dg *op, int32_t n_components, const double *op_order, double
   return OC_OK;
 }
 
-int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const double *factor,
-                              uint32_t n_factor, const double *ri_h, const double *ri_q,
-                              const double *ri_vel, double *const *ki_h, double *const *ki_q,
-                              uint32_t n_ki, double *w_h, double *w_q, double *r_h,
-                              double *r_q, int32_t write_scratch, void *stream)
+// the stage with its per-batch scratch (sweep counts, deferral masks) at `sweeps` / `defer`
+// (null: the operator's own buffers)
+static int stage_impl(oc_dg *op, uint32_t stage, double time, const double *factor,
+                      uint32_t n_factor, const double *ri_h, const double *ri_q,
+                      const double *ri_vel, double *const *ki_h, double *const *ki_q,
+                      uint32_t n_ki, double *w_h, double *w_q, double *r_h, double *r_q,
+                      int32_t write_scratch, void *stream, unsigned int *sweeps,
+                      unsigned long long *defer)
 {
   if (!op || !factor || !ri_h || !ri_q || !ki_h || !ki_q || !w_h || !w_q)
     {
@@ -1134,6 +1189,10 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
         return OC_ERR_ARGUMENT;
       }
   OpArgs O = op->args();
+  if (sweeps)
+    O.sweeps = sweeps;
+  if (defer)
+    O.defer = defer;
   if (const int rc = boundary_entry(op, time, stage, O, "oc_dg_perform_stage_hydro"))
     return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -1159,6 +1218,54 @@ int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const doub
         return stage_d4(op, O, stage, factor, ri_h, ri_q, ri_vel, ki_h, ki_q, n_ki, w_h,
                                w_q, r_h, r_q, write_scratch, s);
     }
+}
+
+int oc_dg_perform_stage_hydro(oc_dg *op, uint32_t stage, double time, const double *factor,
+                              uint32_t n_factor, const double *ri_h, const double *ri_q,
+                              const double *ri_vel, double *const *ki_h, double *const *ki_q,
+                              uint32_t n_ki, double *w_h, double *w_q, double *r_h,
+                              double *r_q, int32_t write_scratch, void *stream)
+{
+  return stage_impl(op, stage, time, factor, n_factor, ri_h, ri_q, ri_vel, ki_h, ki_q, n_ki,
+                    w_h, w_q, r_h, r_q, write_scratch, stream, nullptr, nullptr);
+}
+
+int oc_dg_perform_stage_hydro_noalloc(oc_dg *op, uint32_t stage, double time,
+                                      const double *factor, uint32_t n_factor,
+                                      const double *ri_h, const double *ri_q,
+                                      const double *ri_vel, double *const *ki_h,
+                                      double *const *ki_q, uint32_t n_ki, double *w_h,
+                                      double *w_q, double *r_h, double *r_q,
+                                      int32_t write_scratch, void *stream)
+{
+  if (!op)
+    {
+      set_error("oc_dg_perform_stage_hydro_noalloc: null operator");
+      return OC_ERR_ARGUMENT;
+    }
+  if (op->NB > OC_NOALLOC_MAX_BATCHES)
+    {
+      set_error("oc_dg_perform_stage_hydro_noalloc: " + std::to_string(op->NB) +
+                " cell batches exceed the scratch of " +
+                std::to_string(OC_NOALLOC_MAX_BATCHES) + " batches (" +
+                std::to_string((long long)OC_NOALLOC_MAX_BATCHES * CB) + " cells)");
+      return OC_ERR_ARGUMENT;
+    }
+  unsigned long long *base = g_scratch[op->device];
+  if (!base)
+    {
+      set_error("oc_dg_perform_stage_hydro_noalloc: scratch address not resolved");
+      return OC_ERR_HIP;
+    }
+  // per-kernel timing would create events: not recorded by this entry point
+  const int timing = op->timing;
+  op->timing = 0;
+  const int rc = stage_impl(op, stage, time, factor, n_factor, ri_h, ri_q, ri_vel, ki_h, ki_q,
+                            n_ki, w_h, w_q, r_h, r_q, write_scratch, stream,
+                            reinterpret_cast<unsigned int *>(base + OC_NOALLOC_MAX_BATCHES),
+                            base);
+  op->timing = timing;
+  return rc;
 }
 
 static int rhs_or_apply(oc_dg *op, bool invert, double time, const double *src_h,

@@ -34,6 +34,7 @@ struct oc_rk
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
+  int graph_steps = 1;                    // RK steps recorded in the graph
   bool graph_ticks = false;               // the graph indexes the table by d_tick
   uint64_t graph_generation = 0;
 };
@@ -197,9 +198,15 @@ int oc_rk_time(const oc_rk *rk, double *t)
 int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity,
                   void *stream)
 {
-  if (!rk || !w_h || !w_q || !stream)
+  return oc_rk_capture_steps(rk, 1, dt, w_h, w_q, velocity, stream);
+}
+
+int oc_rk_capture_steps(oc_rk *rk, int32_t n_steps, double dt, double *w_h, double *w_q,
+                        double *velocity, void *stream)
+{
+  if (!rk || !w_h || !w_q || !stream || n_steps < 1)
     {
-      set_error("oc_rk_capture: null argument (capture needs a non-null stream)");
+      set_error("oc_rk_capture: null argument (capture needs a non-null stream) or no step");
       return OC_ERR_ARGUMENT;
     }
   hipStream_t s = (hipStream_t)stream;
@@ -225,17 +232,21 @@ int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *veloci
     }
   if (ticks)
     op->capture_tick = rk->d_tick;
-  int rc = step(rk, dt, w_h, w_q, velocity, s);
-  op->capture_tick = nullptr;
-  if (!rc && ticks)
+  int rc = OC_OK;
+  for (int i = 0; i < n_steps && !rc; ++i)
     {
-      hipLaunchKernelGGL(k_add_tick, dim3(1), dim3(1), 0, s, rk->d_tick, rk->S);
-      if (hipGetLastError() != hipSuccess)
+      rc = step(rk, dt, w_h, w_q, velocity, s);
+      if (!rc && ticks)
         {
-          set_error("oc_rk_capture: tick kernel launch failed");
-          rc = OC_ERR_HIP;
+          hipLaunchKernelGGL(k_add_tick, dim3(1), dim3(1), 0, s, rk->d_tick, rk->S);
+          if (hipGetLastError() != hipSuccess)
+            {
+              set_error("oc_rk_capture: tick kernel launch failed");
+              rc = OC_ERR_HIP;
+            }
         }
     }
+  op->capture_tick = nullptr;
   const hipError_t e = hipStreamEndCapture(s, &rk->graph);
   op->timing = timing;
   if (rc)
@@ -247,6 +258,7 @@ int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *veloci
       return OC_ERR_HIP;
     }
   rk->graph_dt = dt;
+  rk->graph_steps = n_steps;
   rk->graph_ticks = ticks;
   rk->graph_generation = op->generation;
   return OC_OK;
@@ -267,19 +279,31 @@ int oc_rk_replay(oc_rk *rk, int32_t n, void *stream)
       return OC_ERR_ARGUMENT;
     }
   hipStream_t s = (hipStream_t)stream;
+  const long steps = (long)n * rk->graph_steps;
   if (rk->graph_ticks && n > 0)
     {
-      // the n steps read consecutive table entries from the one at the current time
-      const long e0 = oceano::find_boundary_time(op, rk->t);
+      // the steps read consecutive table entries from the first entry at which the stage
+      // times of a whole step start (a step's last stage time usually repeats as the next
+      // step's first, so the entry of the time alone can be the previous step's last)
       const double dt = rk->graph_dt;
-      bool ok = e0 >= 0 && (size_t)e0 + (size_t)n * rk->S <= op->bt_times.size();
-      for (int r = 0; ok && r < n; ++r)
-        for (int st = 0; ok && st < rk->S; ++st)
-          {
-            const double t = rk->t + r * dt + rk->c[st] * dt;
-            const double tab = op->bt_times[(size_t)e0 + (size_t)r * rk->S + st];
-            ok = std::fabs(tab - t) <= 1e-12 * std::max(1.0, std::fabs(t));
-          }
+      const size_t nt = op->bt_times.size(), S = (size_t)rk->S;
+      auto matches = [&](size_t e, long r, size_t st) {
+        const double t = rk->t + r * dt + rk->c[st] * dt;
+        return std::fabs(op->bt_times[e] - t) <= 1e-12 * std::max(1.0, std::fabs(t));
+      };
+      long e0 = -1;
+      for (size_t k = 0; k + S <= nt && e0 < 0; ++k)
+        {
+          bool all = true;
+          for (size_t st = 0; st < S && all; ++st)
+            all = matches(k + st, 0, st);
+          if (all)
+            e0 = (long)k;
+        }
+      bool ok = e0 >= 0 && (size_t)e0 + (size_t)steps * S <= nt;
+      for (long r = 0; ok && r < steps; ++r)
+        for (size_t st = 0; ok && st < S; ++st)
+          ok = matches((size_t)e0 + (size_t)r * S + st, r, st);
       if (!ok)
         {
           set_error("oc_rk_replay: the boundary table does not list the stage times of the "
@@ -300,7 +324,7 @@ int oc_rk_replay(oc_rk *rk, int32_t n, void *stream)
         set_error("oc_rk_replay: hipGraphLaunch failed");
         return OC_ERR_HIP;
       }
-  rk->t += n * rk->graph_dt;
+  rk->t += steps * rk->graph_dt;
   return OC_OK;
 }
 

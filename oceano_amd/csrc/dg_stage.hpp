@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "dg_kernels.hpp"
+#include "dg_mfma.hpp"
 
 namespace oceano {
 
@@ -937,8 +938,27 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
     }
   phase_barrier();
   const R inv_area = R(1) / area;
-  tensor_inverse<P, R>(T, Racc[0], inv_area);
-  tensor_inverse<P, R>(T, Racc[1], inv_area);
+  bool inverted = false;
+  if constexpr (std::is_same<R, float>::value && P >= 3)
+    if (__ballot(1) == ~0ull)
+      {
+        // M^-1 = K / area on the matrix cores (the batch's own data in LDS is dead by now)
+        static_assert(discharge_lds_words<P, VISC>() >= matvec_lds_words<ND, 2>(),
+                      "LDS of the discharge item too small for the inverse mass");
+        float y[2][ND];
+        wave_matvec<ND, 2>(O.kinv, Racc, y, lds);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int k = 0; k < ND; ++k)
+            Racc[c][k] = y[c][k] * inv_area;
+        inverted = true;
+      }
+  if (!inverted)
+    {
+      tensor_inverse<P, R>(T, Racc[0], inv_area);
+      tensor_inverse<P, R>(T, Racc[1], inv_area);
+    }
   if (O.closure)
     {
       // learned momentum-source correction, already in M^-1 form (k_closure)
@@ -1242,10 +1262,13 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
 // hanging vortex.
 __host__ __device__ constexpr bool defer_depth_solve(int p) { return p >= 4; }
 
+// lds: LDS scratch of the wave for the matrix-core product (null: VALU), used at p >= 3
+// when all 64 lanes take part
 template <int P, bool DEFER>
 __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTables &T,
                                                  const Phys<double> &M, int batch, int lane,
-                                                 double area, const double *r, double *Z)
+                                                 double area, const double *r, double *Z,
+                                                 float *lds)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NG = P + 2;
   const Basis<float> &Tf = basis<float>(P);
@@ -1283,23 +1306,38 @@ __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTa
           hmin = h < hmin ? h : hmin;
         }
     }
-  float rf[ND];
+  float rf[1][ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i)
-    rf[i] = float(r[i]);
+    rf[0][i] = float(r[i]);
   const float ia = float(1.0 / area);
   float delta[ND];
   float dmax = 0.f;
+  bool done = false;
+  if constexpr (P >= 3)
+    if (lds && __ballot(1) == ~0ull)
+      {
+        float jr[1][ND];
+        wave_matvec<ND, 1>(O.jmat, rf, jr, lds);
 #pragma unroll
-  for (int i = 0; i < ND; ++i)
-    {
-      float s = 0.f;
+        for (int i = 0; i < ND; ++i)
+          {
+            delta[i] = jr[0][i] * ia;
+            dmax = fmaxf(dmax, fabsf(delta[i]));
+          }
+        done = true;
+      }
+  if (!done)
 #pragma unroll
-      for (int j = 0; j < ND; ++j)
-        s += O.jmat[i * ND + j] * rf[j];
-      delta[i] = s * ia;
-      dmax = fmaxf(dmax, fabsf(delta[i]));
-    }
+    for (int i = 0; i < ND; ++i)
+      {
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+          s += O.jmat[i * ND + j] * rf[0][j];
+        delta[i] = s * ia;
+        dmax = fmaxf(dmax, fabsf(delta[i]));
+      }
   const bool closed = hmin > float(O.wet_margin) * dmax && hmin > 1e-6f;
   if (DEFER)
     {
@@ -1374,7 +1412,7 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
             F[i] += fj * double(kp[(size_t)i * CB]);
         }
       if (centred<R>())
-        k = mixed_depth_solve<P, false>(O, T, M, batch, lane, area, F, Z);
+        k = mixed_depth_solve<P, false>(O, T, M, batch, lane, area, F, Z, nullptr);
       else
         {
           unsigned long long bits = 0;
@@ -1475,7 +1513,7 @@ __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 // registers for the front (it is also written out as ki_h[stage+1] for later stages)
 template <int P, typename R>
 __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageArgs &S,
-                                                   int batch)
+                                                   int batch, float *lds)
 {
   constexpr int ND = (P + 1) * (P + 1);
   const DegreeTables &T = basis<double>(P);
@@ -1544,7 +1582,8 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
         Z[i] = S.wsol[vix(batch, 1, 0, ND, i, lane)];
       issue_barrier();
       if (centred<R>())
-        k = mixed_depth_solve<P, defer_depth_solve(P)>(O, T, M, batch, lane, area, F, Z);
+        k = mixed_depth_solve<P, defer_depth_solve(P)>(O, T, M, batch, lane, area, F, Z,
+                                                       lds);
       else
         {
           unsigned long long bits = 0;
@@ -1591,10 +1630,12 @@ __global__ __launch_bounds__(CB, (stage_min_waves<P, VISC, R>())) void k_stage(O
 {
   __shared__ R lds[discharge_lds_words<P, VISC>()];
   const int w = logical_batch(blockIdx.x, 2 * O.n_batches, O.xcd_swizzle);
+  static_assert(P < 3 || sizeof(lds) >= sizeof(float) * matvec_lds_words<(P + 1) * (P + 1), 1>(),
+                "LDS of the stage too small for the matrix-core depth increment");
   if (item_kind(w))
     discharge_block<P, VISC, 0, R>(O, Q, w >> 1, lds);
   else
-    height_stage_block<P, R>(O, H, w >> 1);
+    height_stage_block<P, R>(O, H, w >> 1, reinterpret_cast<float *>(lds));
 }
 
 }  // namespace oceano

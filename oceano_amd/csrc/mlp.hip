@@ -69,7 +69,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ p,
   __shared__ float s_df[2][16 * HID];
   __shared__ float s_norm[NNORM];
   __shared__ float s_loss[64];
-  const int l = threadIdx.x, r = l & 15, kq = (l >> 4) * 4;
+  const int l = threadIdx.x, r = l & 15, kq = (l >> 4) * 4, k8 = (l >> 4) * 8;
   for (int i = l; i < NT; i += 64)
     s_wt[i] = wt_g[i];
   for (int i = l; i < NWB; i += 64)
@@ -145,13 +145,9 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ p,
         {
           f32x4 acc{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int kk = 0; kk < OUT / 16; ++kk)
-            {
-              const bf16x4 a = *(const bf16x4 *)(s_d[0] + r * OUT + 16 * kk + kq);
-              const bf16x4 b =
-                *(const bf16x4 *)(s_wb + (W4 - WB0) + (16 * ct + r) * OUT + 16 * kk + kq);
-              acc = mfma(a, b, acc);
-            }
+          for (int kk = 0; kk < OUT / 32; ++kk)
+            acc = mfma32(load8(s_d[0] + r * OUT + 32 * kk + k8),
+                         load8(s_wb + (W4 - WB0) + (16 * ct + r) * OUT + 32 * kk + k8), acc);
 #pragma unroll
           for (int j = 0; j < 4; ++j)
             {
@@ -197,13 +193,10 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ p,
             {
               f32x4 acc{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-              for (int kk = 0; kk < HID / 16; ++kk)
-                {
-                  const bf16x4 a = *(const bf16x4 *)(s_d[cur] + r * HID + 16 * kk + kq);
-                  const bf16x4 b =
-                    *(const bf16x4 *)(s_wb + (Wo - WB0) + (16 * ct + r) * HID + 16 * kk + kq);
-                  acc = mfma(a, b, acc);
-                }
+              for (int kk = 0; kk < HID / 32; ++kk)
+                acc = mfma32(load8(s_d[cur] + r * HID + 32 * kk + k8),
+                             load8(s_wb + (Wo - WB0) + (16 * ct + r) * HID + 32 * kk + k8),
+                             acc);
 #pragma unroll
               for (int j = 0; j < 4; ++j)
                 {

@@ -3,14 +3,19 @@
 // inference (dg_closure.hpp).
 //
 // Network: x (IN, zero padded) -> 3 x [64, GELU] -> y (OUT, zero padded), all products on
-// v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32 accumulation), biases, GELU, loss and Adam
-// in FP32.  Inputs are normalised, (x - mu) / sigma, and outputs de-normalised,
-// y * scale + shift, in FP32 by the kernels.
+// the BF16 matrix cores (BF16 operands, FP32 accumulation), biases, GELU, loss and Adam in
+// FP32.  Inputs are normalised, (x - mu) / sigma, and outputs de-normalised,
+// y * scale + shift, in FP32 by the kernels.  Products whose contraction is 32 or more deep
+// (every forward layer, the backward products dY W^T) use gfx950's
+// v_mfma_f32_16x16x32_bf16; the weight gradients, contracted over a 16-row tile of
+// samples, use v_mfma_f32_16x16x16_bf16.
 //
-// MFMA 16x16x16 operand layout (one wave, lane l, r = l % 16, kq = 4 (l / 16)):
-//   A (16 x 16, row-major i x k): lane l supplies A[r][kq + j], j = 0..3
-//   B (16 x 16, k x n):           lane l supplies B[kq + j][r]
-//   D (16 x 16, FP32):            lane l holds    D[kq + j][r]
+// MFMA operand layouts (one wave, lane l, r = l % 16, g = l / 16):
+//   16x16x16: A (16 x 16, i x k) lane l supplies A[r][4 g + j], j = 0..3; B (k x n) supplies
+//             B[4 g + j][r]
+//   16x16x32: A (16 x 32) lane l supplies A[r][8 g + j], j = 0..7; B (32 x 16) supplies
+//             B[8 g + j][r]
+//   D (16 x 16, FP32, both):            lane l holds D[4 g + j][r], j = 0..3
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,6 +45,7 @@ __device__ inline float output_value(float yn, const float *norm, int o)
 }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
@@ -74,25 +80,34 @@ __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c)
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// acc[ct] = A[16 x K] * W for the N / 16 column tiles; A row-major with leading dimension
-// K in LDS, W given transposed (WT[n][k], leading dimension K).  The k order is fixed, so
-// a row's result does not depend on the other rows of the tile.
+__device__ inline f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// the 8 consecutive BF16 values at p (16-byte aligned) as an MFMA operand
+__device__ inline bf16x8 load8(const bf16 *p) { return *(const bf16x8 *)p; }
+
+// acc[ct] = A[16 x K] * W for the N / 16 column tiles (K a multiple of 32); A row-major with
+// leading dimension K in LDS, W given transposed (WT[n][k], leading dimension K).  The k
+// order is fixed, so a row's result does not depend on the other rows of the tile.
 template <int K, int N>
 __device__ inline void dense_tile(const bf16 *A, const bf16 *WT, f32x4 (&acc)[N / 16])
 {
-  const int l = threadIdx.x & 63, r = l & 15, kq = (l >> 4) * 4;
+  static_assert(K % 32 == 0, "contraction depth must be a multiple of 32");
+  const int l = threadIdx.x & 63, r = l & 15, k8 = (l >> 4) * 8;
 #pragma unroll
   for (int ct = 0; ct < N / 16; ++ct)
     acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kk = 0; kk < K / 16; ++kk)
+  for (int kk = 0; kk < K / 32; ++kk)
     {
-      const bf16x4 a = *(const bf16x4 *)(A + r * K + 16 * kk + kq);
+      const bf16x8 a = load8(A + r * K + 32 * kk + k8);
 #pragma unroll
       for (int ct = 0; ct < N / 16; ++ct)
         {
-          const bf16x4 b = *(const bf16x4 *)(WT + (16 * ct + r) * K + 16 * kk + kq);
-          acc[ct] = mfma(a, b, acc[ct]);
+          const bf16x8 b = load8(WT + (16 * ct + r) * K + 32 * kk + k8);
+          acc[ct] = mfma32(a, b, acc[ct]);
         }
     }
 }

@@ -207,12 +207,14 @@ class DeviceIntegrator:
         _capi.check(_capi.lib().oc_rk_step(self._h, dt, self._p(w_h), self._p(w_q),
                                            self._p(velocity), _stream()), "oc_rk_step")
 
-    def capture(self, dt, w_h, w_q, velocity=None):
-        """Record one step on torch's current stream (must not be the null stream)."""
+    def capture(self, dt, w_h, w_q, velocity=None, n_steps=1):
+        """Record n_steps steps into one graph on torch's current stream (must not be the
+        null stream); each replay runs them all."""
         from . import _capi
         from .operator import _stream
-        _capi.check(_capi.lib().oc_rk_capture(self._h, dt, self._p(w_h), self._p(w_q),
-                                              self._p(velocity), _stream()), "oc_rk_capture")
+        _capi.check(_capi.lib().oc_rk_capture_steps(self._h, n_steps, dt, self._p(w_h),
+                                                    self._p(w_q), self._p(velocity),
+                                                    _stream()), "oc_rk_capture")
 
     def replay(self, n=1):
         from . import _capi

@@ -31,6 +31,7 @@ class ClosureNetwork:
         self.n_params = int(lib().oc_mlp_n_params())
         self.normalization = (np.zeros(n_in), np.ones(n_in), np.zeros(n_out), np.ones(n_out))
         self.output_range = (np.full(n_out, -np.inf), np.full(n_out, np.inf))
+        self.output_gain = 1.0
 
     def __del__(self):
         try:
@@ -57,9 +58,22 @@ class ClosureNetwork:
 
     def set_normalization(self, x_mean, x_std, y_shift, y_scale):
         arrs = [np.ascontiguousarray(v, np.float32) for v in (x_mean, x_std, y_shift, y_scale)]
+        self.normalization = tuple(a.astype(np.float64) for a in arrs)
+        self._push_normalization()
+
+    def _push_normalization(self):
+        # the output gain scales the de-normalised output (y * scale + shift) as a whole
+        xm, xs, ys, yc = self.normalization
+        g = self.output_gain
+        arrs = [np.ascontiguousarray(v, np.float32) for v in (xm, xs, g * ys, g * yc)]
         check(lib().oc_mlp_set_normalization(self._h, *[fptr(a) for a in arrs]),
               "oc_mlp_set_normalization")
-        self.normalization = tuple(a.astype(np.float64) for a in arrs)
+
+    def set_output_gain(self, gain):
+        """multiply the network's (de-normalised) output by `gain`: the a-posteriori
+        calibration of the correction (0 = no correction, the plain model)"""
+        self.output_gain = float(gain)
+        self._push_normalization()
 
     def set_output_range(self, lo, hi):
         """saturation of the normalised output at inference (the trained range)"""
@@ -104,7 +118,8 @@ class ClosureNetwork:
         np.savez(path, params=self.get_params(), n_in=self.n_in, n_out=self.n_out,
                  x_mean=self.normalization[0], x_std=self.normalization[1],
                  y_shift=self.normalization[2], y_scale=self.normalization[3],
-                 y_lo=self.output_range[0], y_hi=self.output_range[1], **extra)
+                 y_lo=self.output_range[0], y_hi=self.output_range[1],
+                 output_gain=self.output_gain, **extra)
 
     @classmethod
     def load(cls, path):
@@ -114,4 +129,6 @@ class ClosureNetwork:
         net.set_normalization(d["x_mean"], d["x_std"], d["y_shift"], d["y_scale"])
         if "y_lo" in d:
             net.set_output_range(d["y_lo"], d["y_hi"])
+        if "output_gain" in d:
+            net.set_output_gain(float(d["output_gain"]))
         return net

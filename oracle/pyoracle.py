@@ -12,25 +12,27 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboceano_oracle.so")
+# the same source built with -O3 and FMA contraction: CPU-baseline timing only
+FAST_LIB_PATH = os.path.join(HERE, "liboceano_oracle_fast.so")
 
 BC = {"wall": 1, "supercritical_inflow": 2, "supercritical_outflow": 3, "height_inflow": 4,
       "discharge_inflow": 5, "absorbing_outflow": 6}
 
 PD = C.POINTER(C.c_double)
 PI = C.POINTER(C.c_int)
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(fast=False):
+    path = FAST_LIB_PATH if fast else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         sig = {
             "ocr_last_error": (C.c_char_p, []),
             "ocr_rule": (None, [C.c_int, C.c_int, PD, PD]),
@@ -65,8 +67,8 @@ def lib():
             f = getattr(L, k)
             f.restype = r
             f.argtypes = a
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _d(a):
@@ -96,7 +98,10 @@ STORE = {"cell_0": 0, "cell_1": 1, "cell_2": 2, "face": 3, "boundary": 4, "bvals
 
 class Oracle:
     def __init__(self, geom, inner, boundary, degree, bcs=None, g=9.81, threshold_wetdry=0.1,
-                 viscosity=0.0):
+                 viscosity=0.0, fast=False):
+        """fast=True: the -O3 / FMA build (CPU-baseline timing; not bit-compatible with the
+        parity build)"""
+        self.L = lib(fast)
         self.geom = np.ascontiguousarray(geom, np.float64)
         self.inner = np.ascontiguousarray(inner, np.int32).astype(np.intc)
         self.boundary = np.ascontiguousarray(boundary, np.int32).astype(np.intc)
@@ -110,12 +115,12 @@ class Oracle:
         for bid, k in (bcs or {}).items():
             kinds[bid] = BC[k] if isinstance(k, str) else int(k)
         prm = np.array([g, threshold_wetdry, viscosity])
-        self._h = lib().ocr_create(degree, self.N, _d(self.geom), self.inner.shape[0],
+        self._h = self.L.ocr_create(degree, self.N, _d(self.geom), self.inner.shape[0],
                                    self.inner.ctypes.data_as(PI), self.boundary.shape[0],
                                    self.boundary.ctypes.data_as(PI), kinds.ctypes.data_as(PI),
                                    _d(prm))
         if not self._h:
-            raise RuntimeError(lib().ocr_last_error().decode())
+            raise RuntimeError(self.L.ocr_last_error().decode())
 
     @classmethod
     def from_mesh(cls, mesh, degree, **kw):
@@ -123,24 +128,24 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().ocr_destroy(self._h)
+            self.L.ocr_destroy(self._h)
             self._h = None
 
     def _check(self, rc, where):
         if rc != 0:
-            raise RuntimeError("%s: %s" % (where, lib().ocr_last_error().decode()))
+            raise RuntimeError("%s: %s" % (where, self.L.ocr_last_error().decode()))
 
     def quadrature_points(self, store):
         n = {"cell_0": self.N * self.nq ** 2, "cell_1": self.N * self.ng ** 2,
              "cell_2": self.N * self.nd, "face": self.inner.shape[0] * self.ng * 2,
              "boundary": self.boundary.shape[0] * self.nq}[store]
         xy = np.zeros((n, 2))
-        self._check(lib().ocr_quadrature_points(self._h, STORE[store], _d(xy)), "points")
+        self._check(self.L.ocr_quadrature_points(self._h, STORE[store], _d(xy)), "points")
         return xy
 
     def set_static(self, store, values):
         v = np.ascontiguousarray(values, np.float64)
-        self._check(lib().ocr_set_static(self._h, STORE[store], _d(v)), "set_static")
+        self._check(self.L.ocr_set_static(self._h, STORE[store], _d(v)), "set_static")
 
     def initialize_data_at_quadrature(self, problem_data):
         data = {"cell_0": problem_data(self.quadrature_points("cell_0")),
@@ -156,18 +161,18 @@ class Oracle:
         self.set_static("bvals", np.ascontiguousarray(v, np.float64).reshape(-1))
 
     def set_lane_width(self, w):
-        lib().ocr_set_lane_width(self._h, w)
+        self.L.ocr_set_lane_width(self._h, w)
 
     def set_threads(self, n):
         """n > 1: OpenMP loops (face sums accumulate atomically, rounding-level changes);
         for CPU-baseline timing only"""
-        lib().ocr_set_threads(self._h, n)
+        self.L.ocr_set_threads(self._h, n)
 
     def set_max_iterations(self, k):
-        lib().ocr_set_max_iterations(self._h, k)
+        self.L.ocr_set_max_iterations(self._h, k)
 
     def last_sweeps(self):
-        return lib().ocr_last_sweeps(self._h)
+        return self.L.ocr_last_sweeps(self._h)
 
     def zeros(self, ncomp):
         return np.zeros(self.N * ncomp * self.nd)
@@ -175,18 +180,18 @@ class Oracle:
     def rhs(self, h, q, v=None):
         v = np.zeros_like(q) if v is None else v
         Lh, Lq = self.zeros(1), self.zeros(2)
-        self._check(lib().ocr_rhs(self._h, _d(h), _d(q), _d(v), _d(Lh), _d(Lq)), "rhs")
+        self._check(self.L.ocr_rhs(self._h, _d(h), _d(q), _d(v), _d(Lh), _d(Lq)), "rhs")
         return Lh, Lq
 
     def cell_mass_height(self, src, dst):
-        lib().ocr_cell_mass_height(self._h, _d(src), _d(dst))
+        self.L.ocr_cell_mass_height(self._h, _d(src), _d(dst))
 
     def inverse_mass_height(self, dst, src):
-        lib().ocr_inverse_mass_height(self._h, _d(dst), _d(src))
+        self.L.ocr_inverse_mass_height(self._h, _d(dst), _d(src))
 
     def inverse_mass_tensor(self, x, ncomp):
         out = np.zeros_like(x)
-        lib().ocr_inverse_mass_tensor(self._h, _d(x), _d(out), ncomp)
+        self.L.ocr_inverse_mass_tensor(self._h, _d(x), _d(out), ncomp)
         return out
 
     def perform_stage_hydro(self, stage, factor, ri_h, ri_q, ri_v, ki_h, ki_q, w_h, w_q, r_h,
@@ -196,14 +201,14 @@ class Oracle:
         ri_v = np.zeros_like(ri_q) if ri_v is None else ri_v
         kh = (PD * n_ki)(*[_d(k) for k in ki_h])
         kq = (PD * n_ki)(*[_d(k) for k in ki_q])
-        self._check(lib().ocr_perform_stage_hydro(self._h, stage, n_ki, _d(fac), _d(ri_h),
+        self._check(self.L.ocr_perform_stage_hydro(self._h, stage, n_ki, _d(fac), _d(ri_h),
                                                   _d(ri_q), _d(ri_v), kh, kq, _d(w_h), _d(w_q),
                                                   _d(r_h), _d(r_q)), "perform_stage_hydro")
 
     def project(self, values, ncomp):
         v = np.ascontiguousarray(values, np.float64)
         out = self.zeros(ncomp)
-        lib().ocr_project_collocation(self._h, ncomp, _d(v), _d(out))
+        self.L.ocr_project_collocation(self._h, ncomp, _d(v), _d(out))
         return out
 
     def project_hydro(self, function, zb_cell2):
@@ -216,7 +221,7 @@ class Oracle:
     def errors(self, zeta, q, exact_fn, t):
         ex = np.ascontiguousarray(exact_fn(self.quadrature_points("cell_0"), t))
         err = np.zeros(7)
-        lib().ocr_errors(self._h, _d(zeta), _d(q), _d(ex), _d(err))
+        self.L.ocr_errors(self._h, _d(zeta), _d(q), _d(ex), _d(err))
         return err
 
     def time_step(self, scheme_rows, w_h, w_q, ki_h, ki_q, r_h, r_q, v=None):
@@ -233,16 +238,16 @@ class Oracle:
     def velocity_field(self, zeta, q):
         """evaluate_velocity_field (OceanDG.h:2600-2662)"""
         vel = self.zeros(2)
-        lib().ocr_velocity_field(self._h, _d(zeta), _d(q), _d(vel))
+        self.L.ocr_velocity_field(self._h, _d(zeta), _d(q), _d(vel))
         return vel
 
     def transport_speed(self, zeta, q):
         """compute_cell_transport_speed (OceanDG.h:2506-2580)"""
-        return lib().ocr_transport_speed(self._h, _d(zeta), _d(q))
+        return self.L.ocr_transport_speed(self._h, _d(zeta), _d(q))
 
     def volume(self, zeta):
         """check_mass cell integral (OceanDG.h:2138-2184): integral of the depth"""
-        return lib().ocr_volume(self._h, _d(zeta))
+        return self.L.ocr_volume(self._h, _d(zeta))
 
     def boundary_state(self, kind, z_m, q_m, n, zb, bv):
         """exterior state (z_p, q_p) of one boundary point and the outflow flag
@@ -250,10 +255,10 @@ class Oracle:
         k = BC[kind] if isinstance(kind, str) else int(kind)
         b = np.ascontiguousarray(bv, np.float64)
         out = np.zeros(3)
-        rc = lib().ocr_boundary_state(self._h, k, z_m, q_m[0], q_m[1], n[0], n[1], zb, _d(b),
+        rc = self.L.ocr_boundary_state(self._h, k, z_m, q_m[0], q_m[1], n[0], n[1], zb, _d(b),
                                       _d(out))
         if rc < 0:
-            raise RuntimeError(lib().ocr_last_error().decode())
+            raise RuntimeError(self.L.ocr_last_error().decode())
         return out[0], out[1:].copy(), bool(rc)
 
     def perform_stage_hydro_ark(self, stage, n_stages, factor, factor_tilde, fm, ri_h, ri_q,
@@ -264,7 +269,7 @@ class Oracle:
         ri_v = np.zeros_like(ri_q) if ri_v is None else ri_v
         kh = (PD * len(ki_h))(*[_d(k) for k in ki_h])
         kq = (PD * len(ki_q))(*[_d(k) for k in ki_q])
-        self._check(lib().ocr_perform_stage_hydro_ark(
+        self._check(self.L.ocr_perform_stage_hydro_ark(
             self._h, stage, n_stages, _d(fac), _d(fat), fm, _d(ri_h), _d(ri_q), _d(ri_v), kh,
             kq, _d(w_h), _d(w_q), _d(r_h), _d(r_q)), "perform_stage_hydro_ark")
 

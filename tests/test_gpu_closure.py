@@ -177,17 +177,26 @@ def test_trained_closure_gain_and_fallback(cuda):
     ev = closure.Sample(2000, n_snap=2)
     print("[closure] trained closure on held-out seed 2000: %d active cells"
           % int(ev.active.sum()), flush=True)
+    gain = net.output_gain
+    net.set_output_gain(1.0)  # a priori: the network's own output
     loss, _ = net.gradients(ev.X, ev.T)
+    net.set_output_gain(gain)
     scale = torch.tensor(net.normalization[3], device=cuda, dtype=torch.float32)
     zero = float(((ev.T / scale) ** 2).mean())
     print("  held-out a-priori loss %.3e, no correction %.3e" % (loss, zero))
     assert loss < zero
-    r = closure.evaluate(ev, net)
+    r = closure.evaluate(ev, net, gains=(1.0,))
     for k, v in r.items():
-        print("  %-8s zeta %.3e hu %.3e" % (k, v[0], v[1]))
+        print("  %-10s zeta %.3e hu %.3e" % (k, v[0], v[1]))
     assert all(np.isfinite(v).all() for v in r.values())
     # the exact correction of the initial state improves on the plain model
     assert r["exact"][0] < r["plain"][0]
+    # a posteriori, with the output gain calibrated on coupled runs of training
+    # bathymetries (closure.calibrate_gain), the network does not degrade the plain model
+    # on a held-out one (gain 0 would be the plain model itself)
+    print("  calibrated output gain %g" % net.output_gain)
+    assert r["learned"][0] <= r["plain"][0] * (1 + 1e-9)
+    assert r["learned"][1] <= r["plain"][1] * (1 + 1e-9)
     # fallback: with the network detached the stage is the plain model
     S = ev.S
     op, _, h, q = problems.setup_operator(S.plain)

@@ -15,4 +15,4 @@ def test_cpp_adapter_time_loop(cuda):
     out = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
-    assert out.stdout.count("bitwise identical") == 2 and "OK" in out.stdout
+    assert out.stdout.count("bitwise identical") == 3 and "OK" in out.stdout

@@ -79,3 +79,26 @@ def test_empty_store_accepts_null_pointers(cuda):
     rc = lib().oc_dg_quadrature_points(op._h, _capi.OC_STORE_CELL_GAUSS, None)
     assert rc == _capi.OC_ERR_ARGUMENT
     assert "null pointer" in lib().oc_last_error().decode()
+
+
+@pytest.mark.gpu
+def test_wrongly_shaped_host_arrays_raise(cuda):
+    """The Python mirror checks the size of every host array handed to a C entry point
+    that reads a fixed count (the stores' points x values, n_boundary x nq x 3), so a
+    wrongly shaped array is an error instead of an out-of-bounds host read."""
+    from oceano_amd import problems
+    from oceano_amd.operator import OceanoOperator
+    P = problems.channel(degree=2, nx=10, ny=2)
+    op = OceanoOperator(P.mesh, 2, boundary_conditions=P.bcs)
+    for store, per in OceanoOperator.VALUES_PER_POINT.items():
+        n = op.n_points(store)
+        with pytest.raises(ValueError, match=store):
+            op.set_static_data(store, np.zeros(n * per + 1))
+        op.set_static_data(store, np.zeros(n * per))
+    nb = op.n_boundary * op.n_q_points_1d * 3
+    with pytest.raises(ValueError, match="boundary values"):
+        op.set_boundary_values(np.zeros(nb - 3))
+    with pytest.raises(ValueError, match="boundary table"):
+        op.set_boundary_table([0.0, 1.0], np.zeros(nb))
+    op.set_boundary_values(np.zeros(nb))
+    op.set_boundary_table([0.0, 1.0], np.zeros(2 * nb))
