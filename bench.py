@@ -219,6 +219,10 @@ def main():
     ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
                     help="time the work-layout options per operator and keep the fastest "
                          "(auto: on for the ensemble)")
+    ap.add_argument("--ensemble-launch", default="auto", choices=["auto", "streams", "single"],
+                    help="several operators: one stream per operator (per-degree launches) "
+                         "or one launch per stage for all of them; auto times both on "
+                         "graph-replayed steps and keeps the faster")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this size 3 times (a kernel of "
                          "known HBM traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -295,16 +299,59 @@ def main():
         # replay stream (the host clock around the region is reported beside it)
         side = torch.cuda.Stream()
         full, rest = divmod(n_steps_timed, S)
+        streams = [side] + [torch.cuda.Stream() for _ in G[1:]]
+        launch = None
+        if multi:
+            # several operators: one stream each, or all in one launch per stage
+            from oceano_amd.integrator import EnsembleIntegrator
+            ens = EnsembleIntegrator([g["op"] for g in G])
+            dts_g = [g["dt"] for g in G]
+            cand = {}
+            for mode in (["streams", "single"] if args.ensemble_launch == "auto"
+                         else [args.ensemble_launch]):
+                torch.cuda.synchronize()
+                with torch.cuda.stream(side):
+                    if mode == "single":
+                        ens.capture(dts_g, [g["w_h"] for g in G], [g["w_q"] for g in G],
+                                    n_steps=2)
+                        ens.replay(1)
+                    else:
+                        for g in G:
+                            g["rk"].capture(g["dt"], g["w_h"], g["w_q"], n_steps=2)
+                            g["rk"].replay(1)
+                side.synchronize()
+                for g in G:
+                    g["steps"] += 2
+                t0 = time.perf_counter()
+                if mode == "single":
+                    with torch.cuda.stream(side):
+                        ens.replay(2)
+                else:
+                    for g, st in zip(G, streams):
+                        with torch.cuda.stream(st):
+                            g["rk"].replay(2)
+                torch.cuda.synchronize()
+                cand[mode] = (time.perf_counter() - t0) / 4 * 1e6
+                for g in G:
+                    g["steps"] += 4
+            launch = min(cand, key=cand.get)
+            tuned.setdefault("launch", {})[precision] = {"chosen": launch,
+                                                         "us_per_step": cand}
         torch.cuda.synchronize()
         with torch.cuda.stream(side):
+            if launch == "single":
+                ens.capture(dts_g, [g["w_h"] for g in G], [g["w_q"] for g in G],
+                            n_steps=max(1, full))
+                ens.replay(1)  # first launch of the graph: outside the timed region
+            else:
+                for g in G:
+                    g["rk"].capture(g["dt"], g["w_h"], g["w_q"], n_steps=max(1, full))
+                    g["rk"].replay(1)  # first launch of the graph: outside the timed region
             for g in G:
-                g["rk"].capture(g["dt"], g["w_h"], g["w_q"], n_steps=max(1, full))
-                g["rk"].replay(1)  # first launch of the graph: outside the timed region
                 g["steps"] += max(1, full)
         side.synchronize()
         # independent operators (the ensemble's degree groups) replay on streams of their
         # own, so that one group's kernels overlap another's tail
-        streams = [side] + [torch.cuda.Stream() for _ in G[1:]]
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         quiesce()
@@ -312,10 +359,15 @@ def main():
         ev0.record(side)
         for st in streams[1:]:
             st.wait_event(ev0)
-        for g, st in zip(G, streams):
+        if launch == "single":
             if full:
-                with torch.cuda.stream(st):
-                    g["rk"].replay(1)
+                with torch.cuda.stream(side):
+                    ens.replay(1)
+        else:
+            for g, st in zip(G, streams):
+                if full:
+                    with torch.cuda.stream(st):
+                        g["rk"].replay(1)
         with torch.cuda.stream(side):
             for k in range(rest):
                 stage(G[0], k)
@@ -425,7 +477,7 @@ def main():
                    "scheme": "classic RK4 (Butcher), stage = step",
                    "dt": [g["dt"] for g in head["groups"]] if multi else head["groups"][0]["dt"],
                    "precision": head_p, "memory": note,
-                   "autotuned": tuned if tune_on else None},
+                   "autotuned": tuned if (tune_on or tuned) else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,

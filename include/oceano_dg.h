@@ -40,6 +40,9 @@ const char *oc_last_error(void);
 
 /* Library tuning options read by later oc_dg_create calls (no reference counterpart: the
  * reference leaves cell batching to deal.II's MatrixFree).
+ *   "waves_per_block" 1 (default), 2 or 4: 64-cell work items per workgroup of the one-pass
+ *                 stage kernel (64, 128 or 256 cells per workgroup).  Results do not
+ *                 depend on it.
  *   "slot_chunk"  cells are grouped by face signature (which faces are the coarse side of
  *                 two subfaces or on the boundary) within chunks of this many 64-cell
  *                 batches of the caller's order; 0 (default) = one XCD's share of the
@@ -230,6 +233,25 @@ int oc_dg_perform_stage_hydro_noalloc(oc_dg *op, uint32_t stage, double time,
                                       double *w_q, double *r_h, double *r_q,
                                       int32_t write_scratch, void *stream);
 
+/* The explicit stage of n_ops operators in one launch (the batched ensemble: instances
+ * grouped into one operator per degree, every group advanced by a single grid).  No
+ * reference counterpart (oceano runs one instance per process).  Entry g of each array is
+ * operator g's argument of oc_dg_perform_stage_hydro (its time, its dt-scaled factor row,
+ * its vectors); n_factor, n_ki and stage are shared.  Results equal n_ops separate
+ * oc_dg_perform_stage_hydro calls.  The
+ * operators must share precision and device, be inviscid, have no closure attached and
+ * write outputs distinct from their inputs (OC_ERR_UNSUPPORTED otherwise: call the
+ * single-operator entry per operator).  At degree 4 in OC_PRECISION_MIXED the FP64 depth
+ * solve of near-floor cells follows as a small launch per such operator. */
+#define OC_MULTI_MAX_OPERATORS 4
+int oc_dg_perform_stage_hydro_multi(int32_t n_ops, oc_dg *const *ops, uint32_t stage,
+                                    const double *time, const double *const *factor,
+                                    uint32_t n_factor,
+                                    const double *const *ri_h, const double *const *ri_q,
+                                    double *const *const *ki_h, double *const *const *ki_q,
+                                    uint32_t n_ki, double *const *w_h, double *const *w_q,
+                                    double *const *r_h, double *const *r_q, void *stream);
+
 /* One additive (IMEX) Runge-Kutta stage: the stiff Manning friction is implicit, the rest
  * of the operator explicit.  Replaces the TIMEINTEGRATOR_ADDITIVERUNGEKUTTA overload of
  * OceanoOperator::perform_stage_hydro (OceanDG.h:196-208, body 1936-2119) as called by
@@ -371,6 +393,19 @@ int oc_rk_capture(oc_rk *rk, double dt, double *w_h, double *w_q, double *veloci
 int oc_rk_capture_steps(oc_rk *rk, int32_t n_steps, double dt, double *w_h, double *w_q,
                         double *velocity, void *stream);
 int oc_rk_replay(oc_rk *rk, int32_t n, void *stream);
+/* A time loop over n_ops operators advanced together, one launch per stage
+ * (oc_dg_perform_stage_hydro_multi; the batched ensemble).  Registers and time per
+ * operator; dt, w_h, w_q are arrays of n_ops entries (each operator keeps its own step, e.g.
+ * the CFL step of its degree).  oc_rk_replay replays a captured multi-operator graph;
+ * oc_rk_registers reports the first operator's single-loop registers, not these, and
+ * oc_rk_time the first operator's time.  Time-dependent boundary tables are not supported
+ * in a multi-operator graph. */
+int oc_rk_create_multi(int32_t n_ops, oc_dg *const *ops, int32_t n_stages, const double *rows,
+                       oc_rk **out);
+int oc_rk_step_multi(oc_rk *rk, const double *dt, double *const *w_h, double *const *w_q,
+                     void *stream);
+int oc_rk_capture_multi(oc_rk *rk, int32_t n_steps, const double *dt, double *const *w_h,
+                        double *const *w_q, void *stream);
 /* device pointers of the integrator's registers (arrays of S+1 and 2 entries; any may be
  * null) */
 int oc_rk_registers(oc_rk *rk, double **ki_h, double **ki_q, double **r_h, double **r_q);

@@ -79,6 +79,14 @@ SIGNATURES = {
     "oc_rk_capture": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
     "oc_rk_replay": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "oc_rk_create_multi": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p), C.c_int32,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
+    "oc_rk_step_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.c_void_p]),
+    "oc_rk_capture_multi": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.c_void_p]),
     "oc_rk_capture_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),
     "oc_rk_set_time": (C.c_int, [C.c_void_p, C.c_double]),
@@ -105,6 +113,13 @@ SIGNATURES = {
     "oc_dg_unpack": (C.c_int, [P, I32, P, P, P]),
     "oc_dg_perform_stage_hydro": (C.c_int, [P, U32, D, PD, U32, P, P, P, C.POINTER(P),
                                             C.POINTER(P), U32, P, P, P, P, I32, P]),
+    "oc_dg_perform_stage_hydro_multi": (C.c_int, [I32, C.POINTER(P), U32, PD,
+                                                  C.POINTER(PD), U32,
+                                                  C.POINTER(P), C.POINTER(P),
+                                                  C.POINTER(C.POINTER(P)),
+                                                  C.POINTER(C.POINTER(P)), U32, C.POINTER(P),
+                                                  C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                                  P]),
     "oc_dg_perform_stage_hydro_noalloc": (C.c_int, [P, U32, D, PD, U32, P, P, P,
                                                     C.POINTER(P), C.POINTER(P), U32, P, P, P,
                                                     P, I32, P]),

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "liboceano_dg.so")
 # (source, extra flags, object name): the per-degree stage source is compiled once per degree
 SOURCES = [("mesh.cpp", [], "mesh"), ("icbc.cpp", [], "icbc"),
            ("dg_operator.hip", [], "dg_operator"), ("dg_rk.hip", [], "dg_rk"),
-           ("mlp.hip", [], "mlp")] + \
+           ("mlp.hip", [], "mlp"), ("dg_stage_multi.hip", [], "dg_stage_multi")] + \
           [("dg_stage_degree.hip", ["-DOC_DEGREE=%d" % p], "dg_stage_p%d" % p)
            for p in (1, 2, 3, 4)]
 

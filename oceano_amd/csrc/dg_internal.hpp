@@ -64,6 +64,8 @@ struct oc_dg
   float *d_kinv = nullptr;  // mixed precision: momentum inverse mass (unit cell)
   unsigned long long *d_defer = nullptr;  // mixed precision: per batch, deferred lanes
   double *d_post = nullptr; // per-batch results of the reductions [NB]
+  // arguments of a multi-operator stage launch this operator leads (written by a kernel)
+  oceano::MultiArgs *d_multi_args = nullptr;
   double wet_margin = 10.0;
   // learned closure (oc_dg_set_closure): network (not owned), per-cell sub-grid statistics
   // [batch][3][lane], modal<->nodal tables [nd][nd], correction buffer [batch][2][nd][lane]
@@ -73,6 +75,7 @@ struct oc_dg
   std::vector<double> cell2_host;
   std::vector<double> zb_gauss_host;  // z_b at the OC_STORE_CELL_GAUSS points
   int xcd_swizzle = 1;
+  int waves_per_block = 1;  // one-pass stage kernel: 64-cell items per workgroup (1, 2, 4)
   // optional per-kernel event timing: kernel id -> recorded (start, stop) pairs
   int timing = 0;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
@@ -182,5 +185,14 @@ OC_DECLARE_DEGREE(1)
 OC_DECLARE_DEGREE(2)
 OC_DECLARE_DEGREE(3)
 OC_DECLARE_DEGREE(4)
+
+// one launch for the stage of several operators (dg_stage_multi.hip)
+int upload_tables_multi(const DegreeTables *t, const Basis<float> *tf);
+int stage_multi(int n, oc_dg *const *ops, const OpArgs *O, uint32_t stage,
+                const double *const *factor, const double *const *ri_h,
+                const double *const *ri_q,
+                double *const *const *ki_h, double *const *const *ki_q, uint32_t n_ki,
+                double *const *w_h, double *const *w_q, double *const *r_h,
+                double *const *r_q, hipStream_t s);
 
 }  // namespace oceano

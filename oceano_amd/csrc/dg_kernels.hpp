@@ -106,6 +106,17 @@ struct StageArgs
   unsigned int *sweeps_blk;  // per-batch sum of Jacobi sweeps (height kernels)
 };
 
+// the arguments of one multi-operator stage launch (dg_stage_multi.hip)
+constexpr int MAXG = OC_MULTI_MAX_OPERATORS;
+struct MultiArgs
+{
+  int n_groups;
+  int start[MAXG + 1];  // first workgroup of each group; start[n_groups] = grid size
+  int degree[MAXG];
+  OpArgs O[MAXG];
+  StageArgs H[MAXG], Q[MAXG];
+};
+
 __host__ __device__ constexpr int nq_of(int p) { return (3 * p) / 2 + 1; }
 
 template <int N1>

@@ -36,20 +36,18 @@ __global__ void k_rk_update(RkArgs A)
 }
 
 
-template <int P, typename R>
-int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *factor,
-                 const double *ri_h, const double *ri_q, const double *ri_v,
-                 double *const *ki_h, double *const *ki_q, uint32_t n_ki, double *w_h,
-                 double *w_q, double *r_h, double *r_q, int write_scratch, hipStream_t s)
+// the arguments of the height and discharge work of one explicit stage
+inline void stage_args(const OpArgs &O, uint32_t stage, const double *factor,
+                       const double *ri_h, const double *ri_q, const double *ri_v,
+                       double *const *ki_h, double *const *ki_q, uint32_t n_ki, double *w_h,
+                       double *w_q, double *r_h, double *r_q, int write_scratch, StageArgs &H,
+                       StageArgs &Q)
 {
   const bool last = (stage == n_ki - 2);
-  const dim3 grid(op->NB), block(CB);
   double *h_dst = last ? w_h : r_h;
   double *q_dst = last ? w_q : r_q;
-  const bool visc = O.nu != 0.0;
-  hipEvent_t ev = nullptr;
   // discharge: L_q -> M^-1 L_q -> ki_q[stage+1]; q_dst = w_q + sum_j a_j ki_q[j+1]
-  StageArgs Q{};
+  Q = StageArgs{};
   Q.src_h = ri_h;
   Q.src_q = ri_q;
   Q.src_v = ri_v;
@@ -63,7 +61,7 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
   Q.wsol = w_q;
   Q.rk_dst = q_dst;
   // height: L_zeta -> ki_h[stage+1]; front; Jacobi -> h_dst
-  StageArgs H{};
+  H = StageArgs{};
   H.src_h = ri_h;
   H.src_q = ri_q;
   H.dst = ki_h[stage + 1];
@@ -76,16 +74,40 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
   H.rk_dst = h_dst;
   H.scratch = write_scratch ? ki_h[0] : nullptr;
   H.sweeps_blk = O.sweeps;
+}
+
+template <int P, typename R>
+int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *factor,
+                 const double *ri_h, const double *ri_q, const double *ri_v,
+                 double *const *ki_h, double *const *ki_q, uint32_t n_ki, double *w_h,
+                 double *w_q, double *r_h, double *r_q, int write_scratch, hipStream_t s)
+{
+  const bool last = (stage == n_ki - 2);
+  const dim3 grid(op->NB), block(CB);
+  double *h_dst = last ? w_h : r_h;
+  double *q_dst = last ? w_q : r_q;
+  const bool visc = O.nu != 0.0;
+  hipEvent_t ev = nullptr;
+  StageArgs H, Q;
+  stage_args(O, stage, factor, ri_h, ri_q, ri_v, ki_h, ki_q, n_ki, w_h, w_q, r_h, r_q,
+             write_scratch, H, Q);
   // the one-pass stage reads neighbour traces of ri while other cells write the outputs:
   // it needs outputs distinct from the inputs (single-stage schemes write w in place)
   const bool alias = (h_dst == ri_h) || (q_dst == ri_q) || (h_dst == ri_q) || (q_dst == ri_h);
   if (!alias)
     {
       timed_begin(op, s, &ev);
+      const int items = 2 * op->NB;
       if (visc)
-        hipLaunchKernelGGL((k_stage<P, true, R>), dim3(2 * op->NB), block, 0, s, O, H, Q);
+        hipLaunchKernelGGL((k_stage<P, true, R, 1>), dim3(items), block, 0, s, O, H, Q);
+      else if (op->waves_per_block == 4)
+        hipLaunchKernelGGL((k_stage<P, false, R, 4>), dim3((items + 3) / 4), dim3(4 * CB), 0,
+                           s, O, H, Q);
+      else if (op->waves_per_block == 2)
+        hipLaunchKernelGGL((k_stage<P, false, R, 2>), dim3((items + 1) / 2), dim3(2 * CB), 0,
+                           s, O, H, Q);
       else
-        hipLaunchKernelGGL((k_stage<P, false, R>), dim3(2 * op->NB), block, 0, s, O, H, Q);
+        hipLaunchKernelGGL((k_stage<P, false, R, 1>), dim3(items), block, 0, s, O, H, Q);
       timed_end(op, s, TK_STAGE, ev);
       HIP_TRY(hipGetLastError());
       if (std::is_same<R, float>::value && defer_depth_solve(P))

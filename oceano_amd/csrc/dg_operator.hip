@@ -250,7 +250,8 @@ int ensure_tables()
     }
   int rc;
   if ((rc = upload_tables_d1(t, tf)) || (rc = upload_tables_d2(t, tf)) ||
-      (rc = upload_tables_d3(t, tf)) || (rc = upload_tables_d4(t, tf)))
+      (rc = upload_tables_d3(t, tf)) || (rc = upload_tables_d4(t, tf)) ||
+      (rc = upload_tables_multi(t, tf)))
     return rc;
   g_tables_ready[dev] = true;
   return OC_OK;
@@ -427,6 +428,7 @@ void free_all(oc_dg *op)
   f(op->d_defer);
   f(op->d_post);
   f(op->d_count);
+  f(op->d_multi_args);
 }
 
 // face point on local face f of cell geometry g at face parameter t in [0,1]
@@ -478,6 +480,7 @@ bool uniform(const std::vector<double> &v, double &val)
 // wave per SIMD); the mixed path keeps the caller's order (its two waves per SIMD hide the
 // divergence and it gains more from neighbour locality) -- measured on the C2 vortex.
 int g_xcd_swizzle = 1;  // batch -> workgroup order: each XCD gets a contiguous batch range
+int g_waves_per_block = 1;  // work items (waves of 64 cells) per workgroup of the stage
 int g_slot_chunk = 0;  // batches per chunk; 0: by precision as above; < 0: the caller's order
 
 int cell_slots(const oc_mesh_desc *mesh, int precision, std::vector<int32_t> &order,
@@ -579,6 +582,16 @@ int oc_set_option(const char *name, int64_t value)
       g_slot_chunk = (int)value;
       return OC_OK;
     }
+  if (std::strcmp(name, "waves_per_block") == 0)
+    {
+      if (value != 1 && value != 2 && value != 4)
+        {
+          set_error("oc_set_option: waves_per_block must be 1, 2 or 4");
+          return OC_ERR_ARGUMENT;
+        }
+      g_waves_per_block = (int)value;
+      return OC_OK;
+    }
   if (std::strcmp(name, "xcd_swizzle") == 0)
     {
       g_xcd_swizzle = value != 0;
@@ -614,6 +627,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
   int rc = OC_OK;
   oc_dg *op = new oc_dg;
   op->xcd_swizzle = g_xcd_swizzle;
+  op->waves_per_block = g_waves_per_block;
   op->P = prm->degree;
   op->n1 = op->P + 1;
   op->nd = op->n1 * op->n1;
@@ -800,7 +814,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
       (rc = upload(&op->d_jmat, jacobi_matrix(op->P, op->max_it))) ||
       (rc = upload(&op->d_kinv, inverse_mass_matrix(op->P))) ||
       (rc = dalloc(&op->d_defer, (size_t)NB)) || (rc = dalloc(&op->d_post, (size_t)NB)) ||
-      (rc = dalloc(&op->d_count, 1)))
+      (rc = dalloc(&op->d_count, 1)) || (rc = dalloc(&op->d_multi_args, 1)))
     {
       free_all(op);
       delete op;
@@ -1294,6 +1308,62 @@ static int rhs_or_apply(oc_dg *op, bool invert, double time, const double *src_h
       default:
         return rhs_apply_d4(op, O, invert, src_h, src_q, src_vel, dst_h, dst_q, s);
     }
+}
+
+int oc_dg_perform_stage_hydro_multi(int32_t n_ops, oc_dg *const *ops, uint32_t stage,
+                                    const double *time, const double *const *factor,
+                                    uint32_t n_factor,
+                                    const double *const *ri_h, const double *const *ri_q,
+                                    double *const *const *ki_h, double *const *const *ki_q,
+                                    uint32_t n_ki, double *const *w_h, double *const *w_q,
+                                    double *const *r_h, double *const *r_q, void *stream)
+{
+  const char *fn = "oc_dg_perform_stage_hydro_multi";
+  if (n_ops < 1 || n_ops > OC_MULTI_MAX_OPERATORS || !ops || !time || !factor || !ri_h || !ri_q ||
+      !ki_h || !ki_q || !w_h || !w_q || !r_h || !r_q)
+    {
+      set_error(std::string(fn) + ": null argument or operator count outside 1.." +
+                std::to_string(OC_MULTI_MAX_OPERATORS));
+      return OC_ERR_ARGUMENT;
+    }
+  if (n_ki < 2 || n_ki > MAXKI || stage + 2 > n_ki || n_factor != stage + 1)
+    {
+      set_error(std::string(fn) + ": stage / n_ki / n_factor inconsistent");
+      return OC_ERR_ARGUMENT;
+    }
+  const bool last = stage == n_ki - 2;
+  OpArgs O[OC_MULTI_MAX_OPERATORS];
+  for (int g = 0; g < n_ops; ++g)
+    {
+      oc_dg *op = ops[g];
+      if (!op || !factor[g] || !ri_h[g] || !ri_q[g] || !ki_h[g] || !ki_q[g] || !w_h[g] ||
+          !w_q[g] || (!last && (!r_h[g] || !r_q[g])))
+        {
+          set_error(std::string(fn) + ": null operator or vector");
+          return OC_ERR_ARGUMENT;
+        }
+      for (uint32_t j = 1; j <= stage + 1; ++j)
+        if (!ki_h[g][j] || !ki_q[g][j])
+          {
+            set_error(std::string(fn) + ": null ki vector");
+            return OC_ERR_ARGUMENT;
+          }
+      double *hd = last ? w_h[g] : r_h[g], *qd = last ? w_q[g] : r_q[g];
+      if (op->precision != ops[0]->precision || op->nu != 0.0 || op->closure_mlp ||
+          op->device != ops[0]->device || hd == ri_h[g] || qd == ri_q[g] || hd == ri_q[g] ||
+          qd == ri_h[g])
+        {
+          set_error(std::string(fn) + ": the operators must share precision and device, be "
+                    "inviscid without a closure, and write outputs distinct from the inputs "
+                    "(call oc_dg_perform_stage_hydro per operator otherwise)");
+          return OC_ERR_UNSUPPORTED;
+        }
+      O[g] = op->args();
+      if (const int rc = boundary_entry(op, time[g], stage, O[g], fn))
+        return rc;
+    }
+  return stage_multi(n_ops, ops, O, stage, factor, ri_h, ri_q, ki_h, ki_q, n_ki, w_h, w_q,
+                     r_h, r_q, (hipStream_t)stream);
 }
 
 int oc_dg_perform_stage_hydro_ark(oc_dg *op, uint32_t stage, double time,

@@ -33,6 +33,18 @@ struct oc_rk
   int *d_tick = nullptr;                  // boundary-table entry of the replayed step
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // a loop over several operators (oc_rk_create_multi): their registers, one launch per
+  // stage for all of them (oc_dg_perform_stage_hydro_multi)
+  struct Regs
+  {
+    std::vector<double *> kh, kq;
+    double *rh[2] = {nullptr, nullptr}, *rq[2] = {nullptr, nullptr};
+  };
+  std::vector<oc_dg *> ops;
+  std::vector<Regs> regs;
+  std::vector<double> ts;                 // multi: the time of each operator's next step
+  std::vector<double> graph_dts;          // multi: the steps recorded per operator
+  std::vector<uint64_t> graph_generations;
   double graph_dt = 0.0;
   int graph_steps = 1;                    // RK steps recorded in the graph
   bool graph_ticks = false;               // the graph indexes the table by d_tick
@@ -62,6 +74,18 @@ void release(oc_rk *rk)
     {
       (void)hipFree(rk->rh[i]);
       (void)hipFree(rk->rq[i]);
+    }
+  for (auto &r : rk->regs)
+    {
+      for (double *p : r.kh)
+        (void)hipFree(p);
+      for (double *p : r.kq)
+        (void)hipFree(p);
+      for (int i = 0; i < 2; ++i)
+        {
+          (void)hipFree(r.rh[i]);
+          (void)hipFree(r.rq[i]);
+        }
     }
   (void)hipFree(rk->d_tick);
 }
@@ -98,6 +122,45 @@ int step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity, hipSt
     }
   if (velocity)
     return oc_dg_velocity(rk->op, w_h, w_q, velocity, s);
+  return OC_OK;
+}
+
+// the S stages of one step of every operator of a multi-operator loop, one launch per stage
+int step_multi(oc_rk *rk, const double *dt, double *const *w_h, double *const *w_q,
+               hipStream_t s)
+{
+  const int S = rk->S, n = (int)rk->ops.size();
+  std::vector<const double *> ri_h(n), ri_q(n), fp(n);
+  std::vector<double *const *> kh(n), kq(n);
+  std::vector<double *> rh(n), rq(n);
+  std::vector<std::vector<double>> f(n);
+  std::vector<double> t(n);
+  for (int st = 0; st < S; ++st)
+    {
+      for (int g = 0; g < n; ++g)
+        {
+          f[g] = rk->rows[st];
+          for (double &v : f[g])
+            v *= dt[g];
+          fp[g] = f[g].data();
+          t[g] = rk->ts[g] + rk->c[st] * dt[g];
+        }
+      for (int g = 0; g < n; ++g)
+        {
+          oc_rk::Regs &r = rk->regs[g];
+          ri_h[g] = st == 0 ? w_h[g] : r.rh[st % 2];
+          ri_q[g] = st == 0 ? w_q[g] : r.rq[st % 2];
+          kh[g] = r.kh.data();
+          kq[g] = r.kq.data();
+          rh[g] = r.rh[(st + 1) % 2];
+          rq[g] = r.rq[(st + 1) % 2];
+        }
+      const int rc = oc_dg_perform_stage_hydro_multi(
+        n, rk->ops.data(), (uint32_t)st, t.data(), fp.data(), (uint32_t)st + 1, ri_h.data(),
+        ri_q.data(), kh.data(), kq.data(), (uint32_t)S + 1, w_h, w_q, rh.data(), rq.data(), s);
+      if (rc)
+        return rc;
+    }
   return OC_OK;
 }
 
@@ -181,6 +244,8 @@ int oc_rk_set_time(oc_rk *rk, double t)
       return OC_ERR_ARGUMENT;
     }
   rk->t = t;
+  for (double &v : rk->ts)
+    v = t;
   return OC_OK;
 }
 
@@ -272,7 +337,10 @@ int oc_rk_replay(oc_rk *rk, int32_t n, void *stream)
       return OC_ERR_ARGUMENT;
     }
   oc_dg *op = rk->op;
-  if (op->generation != rk->graph_generation)
+  bool stale = op->generation != rk->graph_generation;
+  for (size_t g = 0; g < rk->graph_generations.size(); ++g)
+    stale = stale || rk->ops[g]->generation != rk->graph_generations[g];
+  if (stale)
     {
       set_error("oc_rk_replay: the operator's static data, boundary data or closure changed "
                 "since oc_rk_capture; capture again");
@@ -325,6 +393,117 @@ int oc_rk_replay(oc_rk *rk, int32_t n, void *stream)
         return OC_ERR_HIP;
       }
   rk->t += steps * rk->graph_dt;
+  for (size_t g = 0; g < rk->ts.size(); ++g)
+    rk->ts[g] += steps * rk->graph_dts[g];
+  return OC_OK;
+}
+
+int oc_rk_create_multi(int32_t n_ops, oc_dg *const *ops, int32_t n_stages, const double *rows,
+                       oc_rk **out)
+{
+  if (!ops || !out || n_ops < 1 || n_ops > OC_MULTI_MAX_OPERATORS)
+    {
+      set_error("oc_rk_create_multi: null argument or operator count outside 1.." +
+                std::to_string(OC_MULTI_MAX_OPERATORS));
+      return OC_ERR_ARGUMENT;
+    }
+  for (int g = 0; g < n_ops; ++g)
+    if (!ops[g])
+      {
+        set_error("oc_rk_create_multi: null operator");
+        return OC_ERR_ARGUMENT;
+      }
+  oc_rk *rk = nullptr;
+  // the first operator's registers are the single-operator ones (unused by multi steps)
+  if (const int rc = oc_rk_create(ops[0], n_stages, rows, &rk))
+    return rc;
+  rk->ops.assign(ops, ops + n_ops);
+  rk->regs.resize(n_ops);
+  rk->ts.assign(n_ops, 0.0);
+  int rc = OC_OK;
+  for (int g = 0; g < n_ops && !rc; ++g)
+    {
+      oc_rk::Regs &r = rk->regs[g];
+      const size_t n1 = (size_t)ops[g]->NB * oceano::CB * ops[g]->nd, n2 = 2 * n1;
+      r.kh.assign(n_stages + 1, nullptr);
+      r.kq.assign(n_stages + 1, nullptr);
+      for (int j = 0; j <= n_stages && !rc; ++j)
+        rc = zeros(&r.kh[j], n1) ? OC_ERR_HIP : zeros(&r.kq[j], n2);
+      for (int i = 0; i < 2 && !rc; ++i)
+        rc = zeros(&r.rh[i], n1) ? OC_ERR_HIP : zeros(&r.rq[i], n2);
+    }
+  if (rc)
+    {
+      oc_rk_destroy(rk);
+      return rc;
+    }
+  *out = rk;
+  return OC_OK;
+}
+
+int oc_rk_step_multi(oc_rk *rk, const double *dt, double *const *w_h, double *const *w_q,
+                     void *stream)
+{
+  if (!rk || rk->ops.empty() || !dt || !w_h || !w_q)
+    {
+      set_error("oc_rk_step_multi: null argument or not a multi-operator loop");
+      return OC_ERR_ARGUMENT;
+    }
+  const int rc = step_multi(rk, dt, w_h, w_q, (hipStream_t)stream);
+  if (!rc)
+    for (size_t g = 0; g < rk->ts.size(); ++g)
+      rk->ts[g] += dt[g];
+  return rc;
+}
+
+int oc_rk_capture_multi(oc_rk *rk, int32_t n_steps, const double *dt, double *const *w_h,
+                        double *const *w_q, void *stream)
+{
+  if (!rk || rk->ops.empty() || !dt || !w_h || !w_q || !stream || n_steps < 1)
+    {
+      set_error("oc_rk_capture_multi: null argument, null stream, no step or not a "
+                "multi-operator loop");
+      return OC_ERR_ARGUMENT;
+    }
+  for (oc_dg *op : rk->ops)
+    if (op->needs_bdata && !op->bt_any)
+      {
+        set_error("oc_rk_capture_multi: time-dependent boundary data are not supported in a "
+                  "multi-operator graph");
+        return OC_ERR_UNSUPPORTED;
+      }
+  hipStream_t s = (hipStream_t)stream;
+  if (rk->exec)
+    (void)hipGraphExecDestroy(rk->exec);
+  if (rk->graph)
+    (void)hipGraphDestroy(rk->graph);
+  rk->exec = nullptr;
+  rk->graph = nullptr;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    {
+      set_error("oc_rk_capture_multi: hipStreamBeginCapture failed");
+      return OC_ERR_HIP;
+    }
+  int rc = OC_OK;
+  for (int i = 0; i < n_steps && !rc; ++i)
+    rc = step_multi(rk, dt, w_h, w_q, s);
+  const hipError_t e = hipStreamEndCapture(s, &rk->graph);
+  if (rc)
+    return rc;
+  if (e != hipSuccess ||
+      hipGraphInstantiate(&rk->exec, rk->graph, nullptr, nullptr, 0) != hipSuccess)
+    {
+      set_error("oc_rk_capture_multi: graph capture or instantiation failed");
+      return OC_ERR_HIP;
+    }
+  rk->graph_dt = dt[0];
+  rk->graph_dts.assign(dt, dt + rk->ops.size());
+  rk->graph_steps = n_steps;
+  rk->graph_ticks = false;
+  rk->graph_generation = rk->op->generation;
+  rk->graph_generations.clear();
+  for (oc_dg *op : rk->ops)
+    rk->graph_generations.push_back(op->generation);
   return OC_OK;
 }
 

@@ -659,7 +659,7 @@ __global__ __launch_bounds__(CB) void k_height_rhs(OpArgs O, const double *__res
   constexpr int ND = (P + 1) * (P + 1);
   const Basis<R> &T = basis<R>(P);
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (CB - 1);
   if (batch * CB + lane >= O.n_cells)
     return;
   const Phys<R> M = physics<R>(O);
@@ -702,7 +702,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   using DT = typename std::conditional<MODE == 0 || MODE == 3 || MODE == 4, R, double>::type;
   DT *const dst = reinterpret_cast<DT *>(S.dst);
   const Basis<R> &T = basis<R>(P);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (CB - 1);
   const double *__restrict__ src_h = S.src_h;
   const double *__restrict__ src_q = S.src_q;
   const double *gh = src_h + (size_t)batch * ND * CB;
@@ -747,16 +747,9 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
           lds[(4 * ND + k) * CB + lane] = R(vg[k]);
         }
     }
-  if (batch * CB + lane >= O.n_cells)
-    return;
-  const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
-  const LdsVec<R> UX{lds + 3 * ND * CB + lane}, UY{lds + 4 * ND * CB + lane};
-  const Phys<R> M = physics<R>(O);
-  const R g = R(O.g), nu = R(O.nu);
-  const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
-  const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
-  const R ihx = R(1) / hx, ihy = R(1) / hy, area = hx * hy;
-  const R zcr = R(zc);
+  // cells outside the mesh (the tail of the last batch) take part in the wave-wide
+  // matrix-core product of the inverse mass with zeros and store nothing
+  const bool active = batch * CB + lane < O.n_cells;
   R Racc[2][ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k)
@@ -764,171 +757,185 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       Racc[0][k] = 0;
       Racc[1][k] = 0;
     }
-  const size_t c0base = (size_t)batch * NQ * NQ * CB + lane;
-  const FieldV<R> bed = in_vgprs<R>(O.cell0[0]);
-#pragma unroll 1
-  for (int b = 0; b < NQ; ++b)
+  R area = 1;
+  if (active)
     {
-      R zs[N1], zd[N1], xs[N1], ys[N1];
-      R uxs[VISC ? N1 : 1], uxd[VISC ? N1 : 1], uys[VISC ? N1 : 1], uyd[VISC ? N1 : 1];
-      R WSX[N1], WDX[N1], WSY[N1], WDY[N1];
-#pragma unroll
-      for (int i = 0; i < N1; ++i)
+      const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
+      const LdsVec<R> UX{lds + 3 * ND * CB + lane}, UY{lds + 4 * ND * CB + lane};
+      const Phys<R> M = physics<R>(O);
+      const R g = R(O.g), nu = R(O.nu);
+      const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
+      const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
+      const R ihx = R(1) / hx, ihy = R(1) / hy;
+      area = hx * hy;
+      const R zcr = R(zc);
+      const size_t c0base = (size_t)batch * NQ * NQ * CB + lane;
+      const FieldV<R> bed = in_vgprs<R>(O.cell0[0]);
+#pragma unroll 1
+      for (int b = 0; b < NQ; ++b)
         {
-          R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+          R zs[N1], zd[N1], xs[N1], ys[N1];
+          R uxs[VISC ? N1 : 1], uxd[VISC ? N1 : 1], uys[VISC ? N1 : 1], uyd[VISC ? N1 : 1];
+          R WSX[N1], WDX[N1], WSY[N1], WDY[N1];
 #pragma unroll
-          for (int j = 0; j < N1; ++j)
+          for (int i = 0; i < N1; ++i)
             {
-              const R zv = Z[i + N1 * j];
-              s0 += T.Sq[b][j] * zv;
-              s1 += T.Dq[b][j] * zv;
-              s2 += T.Sq[b][j] * QX[i + N1 * j];
-              s3 += T.Sq[b][j] * QY[i + N1 * j];
-            }
-          zs[i] = s0;
-          zd[i] = s1;
-          xs[i] = s2;
-          ys[i] = s3;
-          if (VISC)
-            {
-              R t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+              R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
               for (int j = 0; j < N1; ++j)
                 {
-                  t0 += T.Sq[b][j] * UX[i + N1 * j];
-                  t1 += T.Dq[b][j] * UX[i + N1 * j];
-                  t2 += T.Sq[b][j] * UY[i + N1 * j];
-                  t3 += T.Dq[b][j] * UY[i + N1 * j];
+                  const R zv = Z[i + N1 * j];
+                  s0 += T.Sq[b][j] * zv;
+                  s1 += T.Dq[b][j] * zv;
+                  s2 += T.Sq[b][j] * QX[i + N1 * j];
+                  s3 += T.Sq[b][j] * QY[i + N1 * j];
                 }
-              uxs[i] = t0;
-              uxd[i] = t1;
-              uys[i] = t2;
-              uyd[i] = t3;
-            }
-          WSX[i] = WDX[i] = WSY[i] = WDY[i] = 0;
-        }
+              zs[i] = s0;
+              zd[i] = s1;
+              xs[i] = s2;
+              ys[i] = s3;
+              if (VISC)
+                {
+                  R t0 = 0, t1 = 0, t2 = 0, t3 = 0;
 #pragma unroll
-      for (int a = 0; a < NQ; ++a)
-        {
-          R z = 0, zx = 0, zy = 0, qx = 0, qy = 0;
-#pragma unroll
-          for (int i = 0; i < N1; ++i)
-            {
-              z += T.Sq[a][i] * zs[i];
-              zx += T.Dq[a][i] * zs[i];
-              zy += T.Sq[a][i] * zd[i];
-              qx += T.Sq[a][i] * xs[i];
-              qy += T.Sq[a][i] * ys[i];
+                  for (int j = 0; j < N1; ++j)
+                    {
+                      t0 += T.Sq[b][j] * UX[i + N1 * j];
+                      t1 += T.Dq[b][j] * UX[i + N1 * j];
+                      t2 += T.Sq[b][j] * UY[i + N1 * j];
+                      t3 += T.Dq[b][j] * UY[i + N1 * j];
+                    }
+                  uxs[i] = t0;
+                  uxd[i] = t1;
+                  uys[i] = t2;
+                  uyd[i] = t3;
+                }
+              WSX[i] = WDX[i] = WSY[i] = WDY[i] = 0;
             }
-          z += zcr;
-          zx *= ihx;
-          zy *= ihy;
-          const int q = a + NQ * b;
-          const size_t pi = c0base + (size_t)q * CB;
-          const R zb = fld(bed, pi);
-          const R h = M.depth(z, zb);
-          const R s = M.inv_depth(h);
-          const R vx = qx * s, vy = qy * s;
-          R F00 = qx * vx, F01 = qx * vy, F10 = qy * vx, F11 = qy * vy;
-          if (VISC)
+#pragma unroll
+          for (int a = 0; a < NQ; ++a)
             {
-              R G00 = 0, G01 = 0, G10 = 0, G11 = 0;
+              R z = 0, zx = 0, zy = 0, qx = 0, qy = 0;
 #pragma unroll
               for (int i = 0; i < N1; ++i)
                 {
-                  G00 += T.Dq[a][i] * uxs[i];
-                  G01 += T.Sq[a][i] * uxd[i];
-                  G10 += T.Dq[a][i] * uys[i];
-                  G11 += T.Sq[a][i] * uyd[i];
+                  z += T.Sq[a][i] * zs[i];
+                  zx += T.Dq[a][i] * zs[i];
+                  zy += T.Sq[a][i] * zd[i];
+                  qx += T.Sq[a][i] * xs[i];
+                  qy += T.Sq[a][i] * ys[i];
                 }
-              G00 = h * (G00 * ihx);
-              G01 = h * (G01 * ihy);
-              G10 = h * (G10 * ihx);
-              G11 = h * (G11 * ihy);
-              const R div = G00 + G11;
-              F00 = F00 - nu * (G00 + G00) + nu * div;
-              F01 = F01 - nu * (G01 + G10);
-              F10 = F10 - nu * (G10 + G01);
-              F11 = F11 - nu * (G11 + G11) + nu * div;
-            }
-          const R mgh = -g * h;
-          R Sx = mgh * zx, Sy = mgh * zy;
-          if (O.has_friction)
-            {
-              const R n_m = R(fld(O.cell0[1], pi));
-              const R inv_d = R(1) / (exp(log(h) * R(0.33333333333)));
-              const R hcf = g * n_m * n_m * inv_d * sqrt(vx * vx + vy * vy);
-              Sx -= hcf * vx;
-              Sy -= hcf * vy;
-            }
-          if (O.has_wind)
-            {
-              Sx += R(fld(O.cell0[2], pi));
-              Sy += R(fld(O.cell0[3], pi));
-            }
-          if (O.has_coriolis)
-            {
-              const R fc = R(fld(O.cell0[4], pi));
-              Sx += fc * qy;
-              Sy += -fc * qx;
-            }
-          const R JxW = T.wq[a] * T.wq[b] * area;
-          const R g0x = F00 * ihx * JxW, g0y = F01 * ihy * JxW;
-          const R g1x = F10 * ihx * JxW, g1y = F11 * ihy * JxW;
-          const R v0 = Sx * JxW, v1 = Sy * JxW;
+              z += zcr;
+              zx *= ihx;
+              zy *= ihy;
+              const int q = a + NQ * b;
+              const size_t pi = c0base + (size_t)q * CB;
+              const R zb = fld(bed, pi);
+              const R h = M.depth(z, zb);
+              const R s = M.inv_depth(h);
+              const R vx = qx * s, vy = qy * s;
+              R F00 = qx * vx, F01 = qx * vy, F10 = qy * vx, F11 = qy * vy;
+              if (VISC)
+                {
+                  R G00 = 0, G01 = 0, G10 = 0, G11 = 0;
 #pragma unroll
-          for (int i = 0; i < N1; ++i)
-            {
-              WSX[i] += T.Sq[a][i] * v0 + T.Dq[a][i] * g0x;
-              WDX[i] += T.Sq[a][i] * g0y;
-              WSY[i] += T.Sq[a][i] * v1 + T.Dq[a][i] * g1x;
-              WDY[i] += T.Sq[a][i] * g1y;
+                  for (int i = 0; i < N1; ++i)
+                    {
+                      G00 += T.Dq[a][i] * uxs[i];
+                      G01 += T.Sq[a][i] * uxd[i];
+                      G10 += T.Dq[a][i] * uys[i];
+                      G11 += T.Sq[a][i] * uyd[i];
+                    }
+                  G00 = h * (G00 * ihx);
+                  G01 = h * (G01 * ihy);
+                  G10 = h * (G10 * ihx);
+                  G11 = h * (G11 * ihy);
+                  const R div = G00 + G11;
+                  F00 = F00 - nu * (G00 + G00) + nu * div;
+                  F01 = F01 - nu * (G01 + G10);
+                  F10 = F10 - nu * (G10 + G01);
+                  F11 = F11 - nu * (G11 + G11) + nu * div;
+                }
+              const R mgh = -g * h;
+              R Sx = mgh * zx, Sy = mgh * zy;
+              if (O.has_friction)
+                {
+                  const R n_m = R(fld(O.cell0[1], pi));
+                  const R inv_d = R(1) / (exp(log(h) * R(0.33333333333)));
+                  const R hcf = g * n_m * n_m * inv_d * sqrt(vx * vx + vy * vy);
+                  Sx -= hcf * vx;
+                  Sy -= hcf * vy;
+                }
+              if (O.has_wind)
+                {
+                  Sx += R(fld(O.cell0[2], pi));
+                  Sy += R(fld(O.cell0[3], pi));
+                }
+              if (O.has_coriolis)
+                {
+                  const R fc = R(fld(O.cell0[4], pi));
+                  Sx += fc * qy;
+                  Sy += -fc * qx;
+                }
+              const R JxW = T.wq[a] * T.wq[b] * area;
+              const R g0x = F00 * ihx * JxW, g0y = F01 * ihy * JxW;
+              const R g1x = F10 * ihx * JxW, g1y = F11 * ihy * JxW;
+              const R v0 = Sx * JxW, v1 = Sy * JxW;
+#pragma unroll
+              for (int i = 0; i < N1; ++i)
+                {
+                  WSX[i] += T.Sq[a][i] * v0 + T.Dq[a][i] * g0x;
+                  WDX[i] += T.Sq[a][i] * g0y;
+                  WSY[i] += T.Sq[a][i] * v1 + T.Dq[a][i] * g1x;
+                  WDY[i] += T.Sq[a][i] * g1y;
+                }
             }
+#pragma unroll
+          for (int j = 0; j < N1; ++j)
+#pragma unroll
+            for (int i = 0; i < N1; ++i)
+              {
+                Racc[0][i + N1 * j] += T.Sq[b][j] * WSX[i] + T.Dq[b][j] * WDX[i];
+                Racc[1][i + N1 * j] += T.Sq[b][j] * WSY[i] + T.Dq[b][j] * WDY[i];
+              }
         }
-#pragma unroll
-      for (int j = 0; j < N1; ++j)
-#pragma unroll
-        for (int i = 0; i < N1; ++i)
-          {
-            Racc[0][i + N1 * j] += T.Sq[b][j] * WSX[i] + T.Dq[b][j] * WDX[i];
-            Racc[1][i + N1 * j] += T.Sq[b][j] * WSY[i] + T.Dq[b][j] * WDY[i];
-          }
+      phase_barrier();
+      face_terms<P, 2, R>(
+        O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
+        [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
+          // minus side: -(F(own, nbr) + P(own, nbr; zbm, zbp)); plus side: F(nbr, own) +
+          // P(own, nbr; zbp, zbm), written with selects (see the height flux)
+          const Vec2<R> F = M.adv_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn,
+                                       is_m ? qn : qo, n, zbm, zbp);
+          const Vec2<R> Pr = M.pres_flux(zo, zn, n, is_m ? zbm : zbp, is_m ? zbp : zbm);
+          val[0] = is_m ? -F.x - Pr.x : F.x + Pr.x;
+          val[1] = is_m ? -F.y - Pr.y : F.y + Pr.y;
+        },
+        [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
+          R zp;
+          Vec2<R> qp;
+          const bool at_outflow = boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
+          Vec2<R> F = M.adv_flux(zo, zp, qo, qp, n, zb, zb);
+          const Vec2<R> Pr = M.pres_flux(zo, zp, n, zb, zb);
+          F.x += Pr.x;
+          F.y += Pr.y;
+          if (at_outflow && dot_n(qo, n) < R(-1e-12))
+            F = {R(0), R(0)};
+          val[0] = -F.x;
+          val[1] = -F.y;
+        });
     }
-  phase_barrier();
-  face_terms<P, 2, R>(
-    O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
-    [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
-      // minus side: -(F(own, nbr) + P(own, nbr; zbm, zbp)); plus side: F(nbr, own) +
-      // P(own, nbr; zbp, zbm), written with selects (see the height flux)
-      const Vec2<R> F = M.adv_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn,
-                                   is_m ? qn : qo, n, zbm, zbp);
-      const Vec2<R> Pr = M.pres_flux(zo, zn, n, is_m ? zbm : zbp, is_m ? zbp : zbm);
-      val[0] = is_m ? -F.x - Pr.x : F.x + Pr.x;
-      val[1] = is_m ? -F.y - Pr.y : F.y + Pr.y;
-    },
-    [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
-      R zp;
-      Vec2<R> qp;
-      const bool at_outflow = boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
-      Vec2<R> F = M.adv_flux(zo, zp, qo, qp, n, zb, zb);
-      const Vec2<R> Pr = M.pres_flux(zo, zp, n, zb, zb);
-      F.x += Pr.x;
-      F.y += Pr.y;
-      if (at_outflow && dot_n(qo, n) < R(-1e-12))
-        F = {R(0), R(0)};
-      val[0] = -F.x;
-      val[1] = -F.y;
-    });
   if (MODE == 2 || MODE == 4)
     {
+      if (active)
 #pragma unroll
-      for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int k = 0; k < ND; ++k)
-          dst[vix(batch, 2, c, ND, k, lane)] = DT(Racc[c][k]);
+          for (int k = 0; k < ND; ++k)
+            dst[vix(batch, 2, c, ND, k, lane)] = DT(Racc[c][k]);
       return;
     }
-  if (MODE == 0 && S.scratch)
+  if (MODE == 0 && S.scratch && active)
     {
 #pragma unroll
       for (int c = 0; c < 2; ++c)
@@ -938,27 +945,27 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
     }
   phase_barrier();
   const R inv_area = R(1) / area;
-  bool inverted = false;
   if constexpr (std::is_same<R, float>::value && P >= 3)
-    if (__ballot(1) == ~0ull)
-      {
-        // M^-1 = K / area on the matrix cores (the batch's own data in LDS is dead by now)
-        static_assert(discharge_lds_words<P, VISC>() >= matvec_lds_words<ND, 2>(),
-                      "LDS of the discharge item too small for the inverse mass");
-        float y[2][ND];
-        wave_matvec<ND, 2>(O.kinv, Racc, y, lds);
+    {
+      // M^-1 = K / area on the matrix cores, every lane of the wave taking part (the
+      // batch's own data in LDS is dead by now)
+      static_assert(discharge_lds_words<P, VISC>() >= matvec_lds_words<ND, 2>(),
+                    "LDS of the discharge item too small for the inverse mass");
+      float y[2][ND];
+      wave_matvec<ND, 2>(O.kinv, Racc, y, lds);
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+      for (int c = 0; c < 2; ++c)
 #pragma unroll
-          for (int k = 0; k < ND; ++k)
-            Racc[c][k] = y[c][k] * inv_area;
-        inverted = true;
-      }
-  if (!inverted)
+        for (int k = 0; k < ND; ++k)
+          Racc[c][k] = y[c][k] * inv_area;
+    }
+  else
     {
       tensor_inverse<P, R>(T, Racc[0], inv_area);
       tensor_inverse<P, R>(T, Racc[1], inv_area);
     }
+  if (!active)
+    return;
   if (O.closure)
     {
       // learned momentum-source correction, already in M^-1 form (k_closure)
@@ -1262,13 +1269,38 @@ __device__ __forceinline__ int jacobi_solve(const OpArgs &O, const DegreeTables 
 // hanging vortex.
 __host__ __device__ constexpr bool defer_depth_solve(int p) { return p >= 4; }
 
-// lds: LDS scratch of the wave for the matrix-core product (null: VALU), used at p >= 3
-// when all 64 lanes take part
+// The FP32 stage forms J r on the matrix cores at p >= 3 (J the closed form of the capped
+// sweeps, OpArgs::jmat): every lane of the wave takes part, in uniform control flow, a cell
+// outside the mesh with r = 0, so that a cell's arithmetic does not depend on which other
+// cells share its wave.  Returns false (nothing computed) below degree 3.
+template <int P>
+__device__ __forceinline__ bool depth_increment_mfma(const OpArgs &O, bool active,
+                                                     const double *r, float *lds,
+                                                     float (&jr)[(P + 1) * (P + 1)])
+{
+  constexpr int ND = (P + 1) * (P + 1);
+  if constexpr (P >= 3)
+    {
+      float x[1][ND], y[1][ND];
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        x[0][i] = active ? float(r[i]) : 0.f;
+      wave_matvec<ND, 1>(O.jmat, x, y, lds);
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        jr[i] = y[0][i];
+      return true;
+    }
+  return false;
+}
+
+// jr: J r of this cell computed on the matrix cores by the whole wave (depth_increment_mfma),
+// or null to form it here on the VALU
 template <int P, bool DEFER>
 __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTables &T,
                                                  const Phys<double> &M, int batch, int lane,
                                                  double area, const double *r, double *Z,
-                                                 float *lds)
+                                                 const float *jr)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NG = P + 2;
   const Basis<float> &Tf = basis<float>(P);
@@ -1313,21 +1345,14 @@ __device__ __forceinline__ int mixed_depth_solve(const OpArgs &O, const DegreeTa
   const float ia = float(1.0 / area);
   float delta[ND];
   float dmax = 0.f;
-  bool done = false;
-  if constexpr (P >= 3)
-    if (lds && __ballot(1) == ~0ull)
-      {
-        float jr[1][ND];
-        wave_matvec<ND, 1>(O.jmat, rf, jr, lds);
+  if (jr)
 #pragma unroll
-        for (int i = 0; i < ND; ++i)
-          {
-            delta[i] = jr[0][i] * ia;
-            dmax = fmaxf(dmax, fabsf(delta[i]));
-          }
-        done = true;
+    for (int i = 0; i < ND; ++i)
+      {
+        delta[i] = jr[i] * ia;
+        dmax = fmaxf(dmax, fabsf(delta[i]));
       }
-  if (!done)
+  else
 #pragma unroll
     for (int i = 0; i < ND; ++i)
       {
@@ -1373,7 +1398,7 @@ __device__ __forceinline__ void store_sweeps(unsigned int *blk, int batch, int k
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
     v += __shfl_xor(v, off);
-  if (threadIdx.x == 0)
+  if ((threadIdx.x & (CB - 1)) == 0)
     blk[batch] = v;
 }
 
@@ -1383,18 +1408,19 @@ template <int P, typename R, typename KT>
 __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
 {
   constexpr int ND = (P + 1) * (P + 1);
+  __shared__ float lds[centred<R>() && P >= 3 ? matvec_lds_words<ND, 1>() : 1];
   const DegreeTables &T = basis<double>(P);
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (CB - 1);
   const bool active = batch * CB + lane < O.n_cells;
   const Phys<double> M = physics<double>(O);
   int k = 0;
+  double F[ND], Z[ND], area = 1.0;
   if (active)
     {
       const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
       const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
-      const double area = hx * hy;
-      double F[ND], Z[ND];
+      area = hx * hy;
       const size_t base = (size_t)batch * ND * CB + lane;
       const KT *k0 = reinterpret_cast<const KT *>(S.kprev[0]) + base;
 #pragma unroll
@@ -1411,8 +1437,16 @@ __global__ __launch_bounds__(CB) void k_height_jacobi(OpArgs O, StageArgs S)
           for (int i = 0; i < ND; ++i)
             F[i] += fj * double(kp[(size_t)i * CB]);
         }
+    }
+  float jr[ND];
+  bool have_jr = false;
+  if constexpr (centred<R>())
+    have_jr = depth_increment_mfma<P>(O, active, F, lds, jr);
+  if (active)
+    {
       if (centred<R>())
-        k = mixed_depth_solve<P, false>(O, T, M, batch, lane, area, F, Z, nullptr);
+        k = mixed_depth_solve<P, false>(O, T, M, batch, lane, area, F, Z,
+                                        have_jr ? jr : nullptr);
       else
         {
           unsigned long long bits = 0;
@@ -1440,7 +1474,7 @@ __device__ __forceinline__ void depth_deferred_batch(const OpArgs &O, const Stag
 {
   constexpr int ND = (P + 1) * (P + 1);
   const DegreeTables &T = basis<double>(P);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (CB - 1);
   if (!__any(mine))
     return;
   int k = 0;
@@ -1496,7 +1530,7 @@ constexpr int DEFER_BATCHES = 64;
 template <int P>
 __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (CB - 1);
   const int b0 = blockIdx.x * DEFER_BATCHES;
   const unsigned long long m = b0 + lane < O.n_batches ? O.defer[b0 + lane] : 0ull;
   unsigned long long todo = __ballot(m != 0ull);
@@ -1517,16 +1551,16 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
 {
   constexpr int ND = (P + 1) * (P + 1);
   const DegreeTables &T = basis<double>(P);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (CB - 1);
   const bool active = batch * CB + lane < O.n_cells;
   const Phys<double> M = physics<double>(O);
   int k = 0;
+  double F[ND], Z[ND], area = 1.0;
   if (active)
     {
       const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
       const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
-      const double area = hx * hy;
-      double F[ND];
+      area = hx * hy;
       {
         R Zo[ND], QX[ND], QY[ND];
         const double zc = load_own<P, R>(S.src_h, S.src_q, batch, lane, Zo, QX, QY);
@@ -1576,14 +1610,20 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
           }
       }
       phase_barrier();
-      double Z[ND];
 #pragma unroll
       for (int i = 0; i < ND; ++i)
         Z[i] = S.wsol[vix(batch, 1, 0, ND, i, lane)];
       issue_barrier();
+    }
+  float jr[ND];
+  bool have_jr = false;
+  if constexpr (centred<R>())
+    have_jr = depth_increment_mfma<P>(O, active, F, lds, jr);
+  if (active)
+    {
       if (centred<R>())
         k = mixed_depth_solve<P, defer_depth_solve(P)>(O, T, M, batch, lane, area, F, Z,
-                                                       lds);
+                                                       have_jr ? jr : nullptr);
       else
         {
           unsigned long long bits = 0;
@@ -1621,21 +1661,25 @@ constexpr int stage_min_waves()
 // batch (w = 2b, 2b + 1) still have opposite kinds and sit next to each other.
 __device__ __forceinline__ int item_kind(int w) { return __popc((unsigned)w) & 1; }
 
-// the whole stage in one launch: work items w = (batch, kind) are dealt so that each XCD
-// holds a contiguous range of batches
-template <int P, bool VISC, typename R>
-__global__ __launch_bounds__(CB, (stage_min_waves<P, VISC, R>())) void k_stage(OpArgs O,
-                                                                               StageArgs H,
-                                                                               StageArgs Q)
+// the whole stage in one launch: work items w = (batch, kind), WPB waves (items) per
+// workgroup, each wave on its own LDS slice; workgroups are dealt so that each XCD holds a
+// contiguous range of batches
+template <int P, bool VISC, typename R, int WPB>
+__global__ __launch_bounds__(CB * WPB, (stage_min_waves<P, VISC, R>())) void k_stage(
+  OpArgs O, StageArgs H, StageArgs Q)
 {
-  __shared__ R lds[discharge_lds_words<P, VISC>()];
-  const int w = logical_batch(blockIdx.x, 2 * O.n_batches, O.xcd_swizzle);
-  static_assert(P < 3 || sizeof(lds) >= sizeof(float) * matvec_lds_words<(P + 1) * (P + 1), 1>(),
+  __shared__ R lds[WPB][discharge_lds_words<P, VISC>()];
+  static_assert(P < 3 || sizeof(lds[0]) >= sizeof(float) * matvec_lds_words<(P + 1) * (P + 1), 1>(),
                 "LDS of the stage too small for the matrix-core depth increment");
+  const int n_items = 2 * O.n_batches;
+  const int wave = WPB == 1 ? 0 : (int)(threadIdx.x / CB);
+  const int w = logical_batch(blockIdx.x, (n_items + WPB - 1) / WPB, O.xcd_swizzle) * WPB + wave;
+  if (w >= n_items)
+    return;
   if (item_kind(w))
-    discharge_block<P, VISC, 0, R>(O, Q, w >> 1, lds);
+    discharge_block<P, VISC, 0, R>(O, Q, w >> 1, lds[wave]);
   else
-    height_stage_block<P, R>(O, H, w >> 1, reinterpret_cast<float *>(lds));
+    height_stage_block<P, R>(O, H, w >> 1, reinterpret_cast<float *>(lds[wave]));
 }
 
 }  // namespace oceano

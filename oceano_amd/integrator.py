@@ -246,3 +246,62 @@ class DeviceIntegrator:
                 break
         side.synchronize()
         return steps, change
+
+
+class EnsembleIntegrator:
+    """The device-resident RK loop over several operators advanced together (oc_rk_*_multi):
+    one launch per stage for all of them (the batched ensemble's degree groups, one operator
+    per degree over the union of that degree's instances).  step / capture take lists of
+    solution vectors, one pair per operator; replay runs the captured graph."""
+
+    def __init__(self, ops, scheme="stage_4_order_4"):
+        import ctypes as C
+        from . import _capi
+        ai, bi, _ = _tableau(scheme)
+        rows = [v for s in range(len(bi) - 1) for v in ai[s]] + list(bi)
+        arr = (C.c_double * len(rows))(*rows)
+        handles = (C.c_void_p * len(ops))(*[op._h.value for op in ops])
+        h = C.c_void_p()
+        _capi.check(_capi.lib().oc_rk_create_multi(len(ops), handles, len(bi), arr,
+                                                   C.byref(h)), "oc_rk_create_multi")
+        self._h, self.ops, self.n_stages = h, list(ops), len(bi)
+
+    def __del__(self):
+        from . import _capi
+        if getattr(self, "_h", None):
+            _capi.lib().oc_rk_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def _ptrs(ts):
+        import ctypes as C
+        return (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def set_time(self, t):
+        from . import _capi
+        _capi.check(_capi.lib().oc_rk_set_time(self._h, float(t)), "oc_rk_set_time")
+
+    def _dts(self, dt):
+        import ctypes as C
+        d = list(dt) if hasattr(dt, "__len__") else [dt] * len(self.ops)
+        return (C.c_double * len(d))(*d)
+
+    def step(self, dt, ws_h, ws_q):
+        """dt: one step for all operators or one per operator"""
+        from . import _capi
+        from .operator import _stream
+        _capi.check(_capi.lib().oc_rk_step_multi(self._h, self._dts(dt), self._ptrs(ws_h),
+                                                 self._ptrs(ws_q), _stream()),
+                    "oc_rk_step_multi")
+
+    def capture(self, dt, ws_h, ws_q, n_steps=1):
+        from . import _capi
+        from .operator import _stream
+        _capi.check(_capi.lib().oc_rk_capture_multi(self._h, n_steps, self._dts(dt),
+                                                    self._ptrs(ws_h), self._ptrs(ws_q),
+                                                    _stream()), "oc_rk_capture_multi")
+
+    def replay(self, n=1):
+        from . import _capi
+        from .operator import _stream
+        _capi.check(_capi.lib().oc_rk_replay(self._h, n, _stream()), "oc_rk_replay")

@@ -62,16 +62,21 @@ class ClosureNetwork:
         self._push_normalization()
 
     def _push_normalization(self):
-        # the output gain scales the de-normalised output (y * scale + shift) as a whole
+        # the output gain scales the de-normalised output (y * scale + shift) as a whole; a
+        # zero gain switches the correction off where the network is attached
+        # (OceanoOperator.set_closure), so the kernels keep the unscaled output then
         xm, xs, ys, yc = self.normalization
-        g = self.output_gain
+        g = self.output_gain if self.output_gain > 0 else 1.0
         arrs = [np.ascontiguousarray(v, np.float32) for v in (xm, xs, g * ys, g * yc)]
         check(lib().oc_mlp_set_normalization(self._h, *[fptr(a) for a in arrs]),
               "oc_mlp_set_normalization")
 
     def set_output_gain(self, gain):
-        """multiply the network's (de-normalised) output by `gain`: the a-posteriori
-        calibration of the correction (0 = no correction, the plain model)"""
+        """multiply the network's (de-normalised) output by `gain` >= 0: the a-posteriori
+        calibration of the correction (0 = no correction: an operator given this network
+        runs the plain model)"""
+        if not gain >= 0:
+            raise ValueError("output gain must be >= 0")
         self.output_gain = float(gain)
         self._push_normalization()
 

@@ -289,7 +289,10 @@ class OceanoOperator:
               "oc_dg_set_subgrid_stats")
 
     def set_closure(self, network):
-        """attach a ClosureNetwork (None detaches); the operator keeps a reference"""
+        """attach a ClosureNetwork (None detaches); the operator keeps a reference.  A
+        network whose output gain is 0 is the plain model: nothing is attached."""
+        if network is not None and network.output_gain == 0:
+            network = None
         self._closure = network
         check(lib().oc_dg_set_closure(self._h, network.handle if network else None),
               "oc_dg_set_closure")

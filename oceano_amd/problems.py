@@ -178,8 +178,19 @@ def boundary_table(problem, op, t0, dt, n_steps, c):
     them: entry r S + s holds the data at t0 + r dt + c[s] dt (the stage times of
     ExplicitRungeKuttaIntegrator.h:212-271).  Returns (times, values[n, n_bdry, nq, 3])."""
     times = np.array([t0 + r * dt + cs * dt for r in range(n_steps) for cs in c])
-    values = np.stack([problem.boundary_values(op, t) for t in times])
-    return times, values
+    # a step's last stage time repeats as the next step's first (or as another stage's):
+    # evaluate each time once, so that repeated entries hold identical data whichever of
+    # them a call picks
+    values, first = [], []
+    for t in times:
+        tol = 1e-12 * max(1.0, abs(t))
+        j = next((k for k in first[-2 * len(c):] if abs(times[k] - t) <= tol), None)
+        if j is None:
+            first.append(len(values))
+            values.append(problem.boundary_values(op, t))
+        else:
+            values.append(values[j])
+    return times, np.stack(values)
 
 
 def setup_operator(problem, precision="fp64", device="cuda", max_iteration_height=15):

@@ -2,9 +2,11 @@
 deal.II's MatrixFree picks its own cell batching).
 
 The tunable parameters are the library options read at operator creation:
-  slot_chunk   cell slots grouped by face signature within chunks of this many batches
-               (-1: the caller's order, 0: one XCD's share)
-  xcd_swizzle  each XCD runs a contiguous batch range (1) or the round-robin order (0)
+  slot_chunk       cell slots grouped by face signature within chunks of this many batches
+                   (-1: the caller's order, 0: one XCD's share)
+  xcd_swizzle      each XCD runs a contiguous batch range (1) or the round-robin order (0)
+  waves_per_block  64-cell work items per workgroup of the one-pass stage kernel (1, 2, 4):
+                   64, 128 or 256 cells per workgroup, the LDS of each wave its own slice
 Results do not depend on them (each cell's arithmetic is the same); their speed depends on
 the degree, the mesh and the precision.  `autotune` times graph-replayed RK4 steps of the
 problem for each candidate and returns the fastest setting; `apply_options` sets it for
@@ -15,12 +17,15 @@ import time
 from . import _capi
 
 DEFAULT_CANDIDATES = [
-    {"slot_chunk": -1, "xcd_swizzle": 1},
-    {"slot_chunk": 0, "xcd_swizzle": 1},
-    {"slot_chunk": -1, "xcd_swizzle": 0},
-    {"slot_chunk": 0, "xcd_swizzle": 0},
+    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 1},
+    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 1},
+    {"slot_chunk": -1, "xcd_swizzle": 0, "waves_per_block": 1},
+    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 2},
+    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 4},
+    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 2},
+    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 4},
 ]
-LIBRARY_DEFAULTS = {"slot_chunk": 0, "xcd_swizzle": 1}
+LIBRARY_DEFAULTS = {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 1}
 
 
 def apply_options(opts):
