@@ -178,11 +178,11 @@ class DeviceIntegrator:
         h = C.c_void_p()
         _capi.check(_capi.lib().oc_rk_create(op._h, len(bi), arr, C.byref(h)), "oc_rk_create")
         self._h, self.op, self.n_stages = h, op, len(bi)
+        self._lib = _capi.lib()  # kept for __del__, which may run at interpreter shutdown
 
     def __del__(self):
-        from . import _capi
         if getattr(self, "_h", None):
-            _capi.lib().oc_rk_destroy(self._h)
+            self._lib.oc_rk_destroy(self._h)
             self._h = None
 
     @staticmethod
@@ -265,11 +265,11 @@ class EnsembleIntegrator:
         _capi.check(_capi.lib().oc_rk_create_multi(len(ops), handles, len(bi), arr,
                                                    C.byref(h)), "oc_rk_create_multi")
         self._h, self.ops, self.n_stages = h, list(ops), len(bi)
+        self._lib = _capi.lib()  # kept for __del__, which may run at interpreter shutdown
 
     def __del__(self):
-        from . import _capi
         if getattr(self, "_h", None):
-            _capi.lib().oc_rk_destroy(self._h)
+            self._lib.oc_rk_destroy(self._h)
             self._h = None
 
     @staticmethod

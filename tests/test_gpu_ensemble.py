@@ -90,9 +90,11 @@ def test_single_launch_equals_per_degree_launches(cuda, precision):
     The multi-operator kernel runs the same device functions on the same inputs, but it is
     a different compiled kernel: the backend pairs multiplies with adds into FMAs per
     basic block, and the pairing it picks can differ between the two kernels (measured:
-    identical at degrees 1 and 2, 1-ulp differences in about 4% of the degree-4 values after
-    one step), so the comparison with the per-degree kernels is relative to the largest
-    value, at 1e-13 in FP64 and 1e-8 in mixed precision."""
+    identical at degrees 1 and 2; at degree 4, 1-ulp differences in about 4% of the FP64
+    values after one step, and 4.7e-8 of the largest value, under one FP32 unit in the last
+    place, after three mixed-precision steps), so the comparison with the per-degree kernels
+    is relative to the largest value, at 1e-13 in FP64 and 5e-7 (8 FP32 units in the last
+    place) in mixed precision."""
     from oceano_amd.integrator import EnsembleIntegrator
     groups, inst = problems.ensemble(n_instances=8, seed=3, n_min=200, n_max=3000)
     dt = 1e-4
@@ -116,7 +118,7 @@ def test_single_launch_equals_per_degree_launches(cuda, precision):
     for _ in range(3):
         multi.step(dt, e_h, e_q)
     torch.cuda.synchronize()
-    tol = 1e-13 if precision == "fp64" else 1e-8
+    tol = 1e-13 if precision == "fp64" else 5e-7
     for g in range(len(ops)):
         for x, y in ((e_h[g], ref[g][0]), (e_q[g], ref[g][1])):
             rel = float((x - y).abs().max() / y.abs().max())
