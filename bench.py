@@ -211,6 +211,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slot-chunk", type=int, default=None,
                     help="library option slot_chunk (cell grouping by face signature)")
+    ap.add_argument("--paired", type=int, default=None, choices=[0, 1],
+                    help="library option paired_items (a batch's height and discharge "
+                         "items in one workgroup, faces split between them)")
     ap.add_argument("--ark", action="store_true",
                     help="also time the additive (IMEX) stage on the same workload")
     ap.add_argument("--steady", action="store_true",
@@ -236,6 +239,9 @@ def main():
     from oceano_amd import problems
     from oceano_amd.integrator import DeviceIntegrator, butcher_rows
 
+    if args.paired is not None:
+        from oceano_amd import _capi
+        _capi.check(_capi.lib().oc_set_option(b"paired_items", args.paired), "oc_set_option")
     if args.slot_chunk is not None:
         from oceano_amd import _capi
         _capi.check(_capi.lib().oc_set_option(b"slot_chunk", args.slot_chunk), "oc_set_option")

@@ -76,6 +76,7 @@ struct oc_dg
   std::vector<double> zb_gauss_host;  // z_b at the OC_STORE_CELL_GAUSS points
   int xcd_swizzle = 1;
   int waves_per_block = 1;  // one-pass stage kernel: 64-cell items per workgroup (1, 2, 4)
+  int paired_items = 0;     // inviscid one-pass stage: k_stage_pair (faces split per batch)
   // optional per-kernel event timing: kernel id -> recorded (start, stop) pairs
   int timing = 0;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;

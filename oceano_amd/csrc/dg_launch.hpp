@@ -100,6 +100,8 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
       const int items = 2 * op->NB;
       if (visc)
         hipLaunchKernelGGL((k_stage<P, true, R, 1>), dim3(items), block, 0, s, O, H, Q);
+      else if (op->paired_items)
+        hipLaunchKernelGGL((k_stage_pair<P, R>), grid, dim3(2 * CB), 0, s, O, H, Q);
       else if (op->waves_per_block == 4)
         hipLaunchKernelGGL((k_stage<P, false, R, 4>), dim3((items + 3) / 4), dim3(4 * CB), 0,
                            s, O, H, Q);

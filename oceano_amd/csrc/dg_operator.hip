@@ -481,6 +481,11 @@ bool uniform(const std::vector<double> &v, double &val)
 // divergence and it gains more from neighbour locality) -- measured on the C2 vortex.
 int g_xcd_swizzle = 1;  // batch -> workgroup order: each XCD gets a contiguous batch range
 int g_waves_per_block = 1;  // work items (waves of 64 cells) per workgroup of the stage
+// inviscid one-pass stage: a batch's two items in one workgroup, faces split between them;
+// < 0: by precision -- the mixed path pairs (measured on the C2 vortex: 58.2 -> 52.8 us per
+// stage), FP64 does not (114.4 -> 119.9 us: at one wave per SIMD the pair's barrier costs
+// more than the shared face work saves)
+int g_paired_items = -1;
 int g_slot_chunk = 0;  // batches per chunk; 0: by precision as above; < 0: the caller's order
 
 int cell_slots(const oc_mesh_desc *mesh, int precision, std::vector<int32_t> &order,
@@ -597,6 +602,16 @@ int oc_set_option(const char *name, int64_t value)
       g_xcd_swizzle = value != 0;
       return OC_OK;
     }
+  if (std::strcmp(name, "paired_items") == 0)
+    {
+      if (value < -1 || value > 1)
+        {
+          set_error("oc_set_option: paired_items must be -1, 0 or 1");
+          return OC_ERR_ARGUMENT;
+        }
+      g_paired_items = (int)value;
+      return OC_OK;
+    }
   set_error(std::string("oc_set_option: unknown option ") + name);
   return OC_ERR_ARGUMENT;
 }
@@ -628,6 +643,8 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
   oc_dg *op = new oc_dg;
   op->xcd_swizzle = g_xcd_swizzle;
   op->waves_per_block = g_waves_per_block;
+  op->paired_items =
+    g_paired_items < 0 ? (prm->precision == OC_PRECISION_MIXED ? 1 : 0) : g_paired_items;
   op->P = prm->degree;
   op->n1 = op->P + 1;
   op->nd = op->n1 * op->n1;

@@ -220,7 +220,10 @@ __device__ __forceinline__ void subface_integral(const Basis<R> &T, const FieldV
 // lets lane i take item i.  The FP64 path (one wave per SIMD, cells sorted by face
 // signature so that waves rarely diverge) cannot hide the compacted pass's extra load
 // latency and keeps the second subface in the face loop.
-template <int P, int NC, typename R, typename ZA, typename QA, typename InnerFn, typename BdryFn>
+//
+// FMASK selects the faces integrated (bit f: face f); the other faces contribute nothing.
+template <int P, int NC, typename R, int FMASK = 0xF, typename ZA, typename QA, typename InnerFn,
+          typename BdryFn>
 __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, int batch,
                                            int lane, R hx, R hy, const ZA &Z, const QA &QX,
                                            const QA &QY, double zc,
@@ -235,10 +238,12 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
   const R zcr = R(zc);
   // the face tables of all four faces in one group of loads: each face then waits for one
   // memory latency (its neighbour's traces) instead of three dependent ones
-  int infos[4], nbrs[4][NS], fids[4][NS];
+  int infos[4] = {}, nbrs[4][NS] = {}, fids[4][NS] = {};
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     {
+      if (!((FMASK >> f) & 1))
+        continue;
       const size_t fo = ((size_t)batch * 4 + f);
       infos[f] = O.finfo[fo * CB + lane];
 #pragma unroll
@@ -252,10 +257,12 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
   unsigned long long coarse[4];  // lanes whose face f is the coarse side of a hanging face
 #pragma unroll
   for (int f = 0; f < 4; ++f)
-    coarse[f] = COMPACT ? __ballot((infos[f] & 3) == FK_COARSE) : 0ull;
+    coarse[f] = COMPACT && ((FMASK >> f) & 1) ? __ballot((infos[f] & 3) == FK_COARSE) : 0ull;
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     {
+      if (!((FMASK >> f) & 1))
+        continue;
       phase_barrier();
       const int info = infos[f];
       const int kind = info & 3;
@@ -537,22 +544,128 @@ __device__ __forceinline__ void tensor_inverse(const Basis<R> &T, R *U, R inv_ar
 }
 
 // ---------------------------------------------------------------------------------------
-// L_zeta of one cell (cell term + faces) into Racc
+// Face fluxes of the two equations, as the `inner` / `bdry` functions of face_terms
 // ---------------------------------------------------------------------------------------
-template <int P, typename R, typename ZA, typename QA>
-__device__ __forceinline__ void height_operator(const OpArgs &O, const Basis<R> &T,
-                                                const Phys<R> &M, int batch, int lane, R hx,
-                                                R hy, const ZA &Z, const QA &QX, const QA &QY,
-                                                double zc, const double *__restrict__ src_h,
-                                                const double *__restrict__ src_q,
-                                                R (&Racc)[1][(P + 1) * (P + 1)])
+// mass: minus side -F(own, neighbour); plus side F(neighbour, own).  Selects instead of
+// branches: the face side is a constant in the per-face loop and a lane value in the
+// compacted second-subface pass
+template <typename R>
+__device__ __forceinline__ R mass_face(const Phys<R> &M, bool is_m, R zo, Vec2<R> qo, R zn,
+                                       Vec2<R> qn, Vec2<R> n, R zbm, R zbp)
 {
-  constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
+  const R f = M.mass_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn, is_m ? qn : qo, n,
+                          zbm, zbp);
+  return is_m ? -f : f;
+}
+
+// momentum: minus side -(F(own, nbr) + P(own, nbr; zbm, zbp)); plus side F(nbr, own) +
+// P(own, nbr; zbp, zbm)
+template <typename R>
+__device__ __forceinline__ Vec2<R> momentum_face(const Phys<R> &M, bool is_m, R zo, Vec2<R> qo,
+                                                 R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp)
+{
+  const Vec2<R> F = M.adv_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn, is_m ? qn : qo,
+                               n, zbm, zbp);
+  const Vec2<R> Pr = M.pres_flux(zo, zn, n, is_m ? zbm : zbp, is_m ? zbp : zbm);
+  return {is_m ? -F.x - Pr.x : F.x + Pr.x, is_m ? -F.y - Pr.y : F.y + Pr.y};
+}
+
+// boundary face: exterior state of the condition (OceanDG.h:1130-1203 / 1246-1317), mass
+// and momentum flux; the momentum flux is cut where an outflow boundary sees backflow
+// (OceanDG.h:1326-1332)
+template <typename R>
+__device__ __forceinline__ void boundary_faces(const Phys<R> &M, int bk, R zo, Vec2<R> qo,
+                                               Vec2<R> n, R zb, const double *bv, R &mass,
+                                               Vec2<R> &mom)
+{
+  R zp;
+  Vec2<R> qp;
+  const bool at_outflow = boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
+  mass = -M.mass_flux(zo, zp, qo, qp, n, zb, zb);
+  Vec2<R> F = M.adv_flux(zo, zp, qo, qp, n, zb, zb);
+  const Vec2<R> Pr = M.pres_flux(zo, zp, n, zb, zb);
+  F.x += Pr.x;
+  F.y += Pr.y;
+  if (at_outflow && dot_n(qo, n) < R(-1e-12))
+    F = {R(0), R(0)};
+  mom = {-F.x, -F.y};
+}
+
+template <typename R>
+struct MassFaces
+{
+  Phys<R> M;
+  __device__ __forceinline__ void operator()(bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn,
+                                             Vec2<R> n, R zbm, R zbp, R *val) const
+  {
+    val[0] = mass_face(M, is_m, zo, qo, zn, qn, n, zbm, zbp);
+  }
+  __device__ __forceinline__ void operator()(int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb,
+                                             const double *bv, R *val) const
+  {
+    R zp;
+    Vec2<R> qp;
+    boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
+    val[0] = -M.mass_flux(zo, zp, qo, qp, n, zb, zb);
+  }
+};
+
+template <typename R>
+struct MomentumFaces
+{
+  Phys<R> M;
+  __device__ __forceinline__ void operator()(bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn,
+                                             Vec2<R> n, R zbm, R zbp, R *val) const
+  {
+    const Vec2<R> f = momentum_face(M, is_m, zo, qo, zn, qn, n, zbm, zbp);
+    val[0] = f.x;
+    val[1] = f.y;
+  }
+  __device__ __forceinline__ void operator()(int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb,
+                                             const double *bv, R *val) const
+  {
+    R mass;
+    Vec2<R> mom;
+    boundary_faces(M, bk, zo, qo, n, zb, bv, mass, mom);
+    val[0] = mom.x;
+    val[1] = mom.y;
+  }
+};
+
+// both equations at once (val[0] mass, val[1], val[2] momentum): the traces, velocities,
+// wave speeds and reconstructed depths of a face point are formed once for the three
+template <typename R>
+struct AllFaces
+{
+  Phys<R> M;
+  __device__ __forceinline__ void operator()(bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn,
+                                             Vec2<R> n, R zbm, R zbp, R *val) const
+  {
+    val[0] = mass_face(M, is_m, zo, qo, zn, qn, n, zbm, zbp);
+    const Vec2<R> f = momentum_face(M, is_m, zo, qo, zn, qn, n, zbm, zbp);
+    val[1] = f.x;
+    val[2] = f.y;
+  }
+  __device__ __forceinline__ void operator()(int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb,
+                                             const double *bv, R *val) const
+  {
+    R mass;
+    Vec2<R> mom;
+    boundary_faces(M, bk, zo, qo, n, zb, bv, mass, mom);
+    val[0] = mass;
+    val[1] = mom.x;
+    val[2] = mom.y;
+  }
+};
+
+// cell term of L_zeta: acc += integral of grad(phi) . q (mass_flux = q,
+// ShallowWater.h:352-357)
+template <int P, typename R, typename QA>
+__device__ __forceinline__ void height_cell_term(const Basis<R> &T, R hx, R hy, const QA &QX,
+                                                 const QA &QY, R (&acc)[(P + 1) * (P + 1)])
+{
+  constexpr int N1 = P + 1, NQ = nq_of(P);
   const R ihx = R(1) / hx, ihy = R(1) / hy, area = hx * hy;
-#pragma unroll
-  for (int k = 0; k < ND; ++k)
-    Racc[0][k] = 0;
-  // cell term: integral of grad(phi) . q  (mass_flux = q, ShallowWater.h:352-357)
 #pragma unroll 1
   for (int b = 0; b < NQ; ++b)
     {
@@ -595,25 +708,29 @@ __device__ __forceinline__ void height_operator(const OpArgs &O, const Basis<R> 
       for (int j = 0; j < N1; ++j)
 #pragma unroll
         for (int i = 0; i < N1; ++i)
-          Racc[0][i + N1 * j] += T.Sq[b][j] * WS[i] + T.Dq[b][j] * WD[i];
+          acc[i + N1 * j] += T.Sq[b][j] * WS[i] + T.Dq[b][j] * WD[i];
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// L_zeta of one cell (cell term + faces) into Racc
+// ---------------------------------------------------------------------------------------
+template <int P, typename R, typename ZA, typename QA>
+__device__ __forceinline__ void height_operator(const OpArgs &O, const Basis<R> &T,
+                                                const Phys<R> &M, int batch, int lane, R hx,
+                                                R hy, const ZA &Z, const QA &QX, const QA &QY,
+                                                double zc, const double *__restrict__ src_h,
+                                                const double *__restrict__ src_q,
+                                                R (&Racc)[1][(P + 1) * (P + 1)])
+{
+  constexpr int ND = (P + 1) * (P + 1);
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+    Racc[0][k] = 0;
+  height_cell_term<P, R>(T, hx, hy, QX, QY, Racc[0]);
   phase_barrier();
-  face_terms<P, 1, R>(
-    O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
-    [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
-      // minus side: -F(own, neighbour); plus side: F(neighbour, own).  Selects instead of
-      // branches: the face side is a constant in the per-face loop and a lane value in the
-      // compacted second-subface pass
-      const R f = M.mass_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn, is_m ? qn : qo,
-                              n, zbm, zbp);
-      val[0] = is_m ? -f : f;
-    },
-    [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
-      R zp;
-      Vec2<R> qp;
-      boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
-      val[0] = -M.mass_flux(zo, zp, qo, qp, n, zb, zb);
-    });
+  const MassFaces<R> fn{M};
+  face_terms<P, 1, R>(O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc, fn, fn);
 }
 
 // own data of a cell in registers: Z as deviation from zc (zc = 0 for FP64)
@@ -693,9 +810,223 @@ constexpr int discharge_lds_words()
   return (VISC ? 5 : 3) * (P + 1) * (P + 1) * CB;
 }
 
-template <int P, bool VISC, int MODE, typename R>
+// ---------------------------------------------------------------------------------------
+// Paired work items: the height wave (ROLE 0) and the discharge wave (ROLE 1) of one batch
+// share a workgroup and split the faces instead of each integrating all four for its own
+// equation: ROLE 0 integrates faces 0 and 1 (x = const), ROLE 1 faces 2 and 3 (y = const),
+// each for the mass and both momentum components, so that every face point's traces,
+// velocities, wave speeds and reconstructed depths are formed once per cell instead of
+// twice.  pair_exchange then hands over what belongs to the other wave.  A[0] (mass) and
+// A[1], A[2] (momentum) hold, on entry, this wave's cell term and its two faces'
+// integrals; on return the wave's own equation holds all four faces.  A face integral
+// touches only the cell's N1 DoFs on that face, so the swap moves 4 N1 momentum values
+// (ROLE 0 -> 1) and 2 N1 mass values (ROLE 1 -> 0) per cell through `xch`
+// (pair_xch_words() words of LDS).  Both waves call it once, every lane (one workgroup
+// barrier).
+// ---------------------------------------------------------------------------------------
+template <int P>
+__host__ __device__ constexpr int pair_xch_words()
+{
+  return 6 * (P + 1) * CB;
+}
+
+template <int P, typename R, int ROLE>
+__device__ __forceinline__ void pair_exchange(R (&A)[3][(P + 1) * (P + 1)], R *xch)
+{
+  constexpr int N1 = P + 1;
+  const int lane = threadIdx.x & (CB - 1);
+  // slots: momentum component c of face f < 2 at ((c - 1) 2 + f) N1 + k; mass of face
+  // f >= 2 at 4 N1 + (f - 2) N1 + k
+  if constexpr (ROLE == 0)
+    {
+#pragma unroll
+      for (int c = 1; c < 3; ++c)
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+          for (int k = 0; k < N1; ++k)
+            xch[(((c - 1) * 2 + f) * N1 + k) * CB + lane] = A[c][face_dof<N1>(f, k)];
+    }
+  else
+    {
+#pragma unroll
+      for (int f = 2; f < 4; ++f)
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+          xch[(4 * N1 + (f - 2) * N1 + k) * CB + lane] = A[0][face_dof<N1>(f, k)];
+    }
+  __syncthreads();
+  if constexpr (ROLE == 0)
+    {
+#pragma unroll
+      for (int f = 2; f < 4; ++f)
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+          A[0][face_dof<N1>(f, k)] += xch[(4 * N1 + (f - 2) * N1 + k) * CB + lane];
+    }
+  else
+    {
+#pragma unroll
+      for (int c = 1; c < 3; ++c)
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+          for (int k = 0; k < N1; ++k)
+            A[c][face_dof<N1>(f, k)] += xch[(((c - 1) * 2 + f) * N1 + k) * CB + lane];
+    }
+}
+
+// cell term of L_q (advective and diffusive flux, gravity, Manning friction, wind stress,
+// Coriolis; OceanDG.h:696-743, ShallowWater.h:381-432): AX, AY += the integrals against
+// grad(phi) and phi.  The batch's own zeta (deviation from zcr), q (and velocity) are read
+// from the lane-interleaved LDS copy `lds`.
+template <int P, bool VISC, typename R>
+__device__ __forceinline__ void discharge_cell_term(const OpArgs &O, const Basis<R> &T,
+                                                    const R *lds, int batch, int lane, R hx,
+                                                    R hy, R zcr, R (&AX)[(P + 1) * (P + 1)],
+                                                    R (&AY)[(P + 1) * (P + 1)])
+{
+  constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
+  const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
+  const LdsVec<R> UX{lds + 3 * ND * CB + lane}, UY{lds + 4 * ND * CB + lane};
+  const Phys<R> M = physics<R>(O);
+  const R g = R(O.g), nu = R(O.nu);
+  const R ihx = R(1) / hx, ihy = R(1) / hy, area = hx * hy;
+  const size_t c0base = (size_t)batch * NQ * NQ * CB + lane;
+  const FieldV<R> bed = in_vgprs<R>(O.cell0[0]);
+#pragma unroll 1
+  for (int b = 0; b < NQ; ++b)
+    {
+      R zs[N1], zd[N1], xs[N1], ys[N1];
+      R uxs[VISC ? N1 : 1], uxd[VISC ? N1 : 1], uys[VISC ? N1 : 1], uyd[VISC ? N1 : 1];
+      R WSX[N1], WDX[N1], WSY[N1], WDY[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i)
+        {
+          R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+          for (int j = 0; j < N1; ++j)
+            {
+              const R zv = Z[i + N1 * j];
+              s0 += T.Sq[b][j] * zv;
+              s1 += T.Dq[b][j] * zv;
+              s2 += T.Sq[b][j] * QX[i + N1 * j];
+              s3 += T.Sq[b][j] * QY[i + N1 * j];
+            }
+          zs[i] = s0;
+          zd[i] = s1;
+          xs[i] = s2;
+          ys[i] = s3;
+          if (VISC)
+            {
+              R t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+              for (int j = 0; j < N1; ++j)
+                {
+                  t0 += T.Sq[b][j] * UX[i + N1 * j];
+                  t1 += T.Dq[b][j] * UX[i + N1 * j];
+                  t2 += T.Sq[b][j] * UY[i + N1 * j];
+                  t3 += T.Dq[b][j] * UY[i + N1 * j];
+                }
+              uxs[i] = t0;
+              uxd[i] = t1;
+              uys[i] = t2;
+              uyd[i] = t3;
+            }
+          WSX[i] = WDX[i] = WSY[i] = WDY[i] = 0;
+        }
+#pragma unroll
+      for (int a = 0; a < NQ; ++a)
+        {
+          R z = 0, zx = 0, zy = 0, qx = 0, qy = 0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i)
+            {
+              z += T.Sq[a][i] * zs[i];
+              zx += T.Dq[a][i] * zs[i];
+              zy += T.Sq[a][i] * zd[i];
+              qx += T.Sq[a][i] * xs[i];
+              qy += T.Sq[a][i] * ys[i];
+            }
+          z += zcr;
+          zx *= ihx;
+          zy *= ihy;
+          const int q = a + NQ * b;
+          const size_t pi = c0base + (size_t)q * CB;
+          const R zb = fld(bed, pi);
+          const R h = M.depth(z, zb);
+          const R s = M.inv_depth(h);
+          const R vx = qx * s, vy = qy * s;
+          R F00 = qx * vx, F01 = qx * vy, F10 = qy * vx, F11 = qy * vy;
+          if (VISC)
+            {
+              R G00 = 0, G01 = 0, G10 = 0, G11 = 0;
+#pragma unroll
+              for (int i = 0; i < N1; ++i)
+                {
+                  G00 += T.Dq[a][i] * uxs[i];
+                  G01 += T.Sq[a][i] * uxd[i];
+                  G10 += T.Dq[a][i] * uys[i];
+                  G11 += T.Sq[a][i] * uyd[i];
+                }
+              G00 = h * (G00 * ihx);
+              G01 = h * (G01 * ihy);
+              G10 = h * (G10 * ihx);
+              G11 = h * (G11 * ihy);
+              const R div = G00 + G11;
+              F00 = F00 - nu * (G00 + G00) + nu * div;
+              F01 = F01 - nu * (G01 + G10);
+              F10 = F10 - nu * (G10 + G01);
+              F11 = F11 - nu * (G11 + G11) + nu * div;
+            }
+          const R mgh = -g * h;
+          R Sx = mgh * zx, Sy = mgh * zy;
+          if (O.has_friction)
+            {
+              const R n_m = R(fld(O.cell0[1], pi));
+              const R inv_d = R(1) / (exp(log(h) * R(0.33333333333)));
+              const R hcf = g * n_m * n_m * inv_d * sqrt(vx * vx + vy * vy);
+              Sx -= hcf * vx;
+              Sy -= hcf * vy;
+            }
+          if (O.has_wind)
+            {
+              Sx += R(fld(O.cell0[2], pi));
+              Sy += R(fld(O.cell0[3], pi));
+            }
+          if (O.has_coriolis)
+            {
+              const R fc = R(fld(O.cell0[4], pi));
+              Sx += fc * qy;
+              Sy += -fc * qx;
+            }
+          const R JxW = T.wq[a] * T.wq[b] * area;
+          const R g0x = F00 * ihx * JxW, g0y = F01 * ihy * JxW;
+          const R g1x = F10 * ihx * JxW, g1y = F11 * ihy * JxW;
+          const R v0 = Sx * JxW, v1 = Sy * JxW;
+#pragma unroll
+          for (int i = 0; i < N1; ++i)
+            {
+              WSX[i] += T.Sq[a][i] * v0 + T.Dq[a][i] * g0x;
+              WDX[i] += T.Sq[a][i] * g0y;
+              WSY[i] += T.Sq[a][i] * v1 + T.Dq[a][i] * g1x;
+              WDY[i] += T.Sq[a][i] * g1y;
+            }
+        }
+#pragma unroll
+      for (int j = 0; j < N1; ++j)
+#pragma unroll
+        for (int i = 0; i < N1; ++i)
+          {
+            AX[i + N1 * j] += T.Sq[b][j] * WSX[i] + T.Dq[b][j] * WDX[i];
+            AY[i + N1 * j] += T.Sq[b][j] * WSY[i] + T.Dq[b][j] * WDY[i];
+          }
+    }
+}
+
+template <int P, bool VISC, int MODE, typename R, bool PAIR = false>
 __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs &S, int batch,
-                                                R *lds)
+                                                R *lds, R *xch = nullptr)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
   // dst of a stage is a k vector (stored in R); of apply / rhs a caller vector (FP64)
@@ -758,172 +1089,47 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       Racc[1][k] = 0;
     }
   R area = 1;
-  if (active)
+  if constexpr (PAIR)
+    {
+      // faces 2 and 3 for both equations (pair_exchange)
+      static_assert(!VISC && MODE == 0, "paired items: inviscid explicit stage only");
+      R A[3][ND];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < ND; ++k)
+          A[c][k] = 0;
+      if (active)
+        {
+          const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane},
+            QY{lds + 2 * ND * CB + lane};
+          const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
+          const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
+          area = hx * hy;
+          discharge_cell_term<P, VISC, R>(O, T, lds, batch, lane, hx, hy, R(zc), A[1], A[2]);
+          phase_barrier();
+          const AllFaces<R> fn{physics<R>(O)};
+          face_terms<P, 3, R, 0xC>(O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, A,
+                                   fn, fn);
+        }
+      pair_exchange<P, R, 1>(A, xch);
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < ND; ++k)
+          Racc[c][k] = A[1 + c][k];
+    }
+  else if (active)
     {
       const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
-      const LdsVec<R> UX{lds + 3 * ND * CB + lane}, UY{lds + 4 * ND * CB + lane};
       const Phys<R> M = physics<R>(O);
-      const R g = R(O.g), nu = R(O.nu);
       const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
       const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
-      const R ihx = R(1) / hx, ihy = R(1) / hy;
       area = hx * hy;
-      const R zcr = R(zc);
-      const size_t c0base = (size_t)batch * NQ * NQ * CB + lane;
-      const FieldV<R> bed = in_vgprs<R>(O.cell0[0]);
-#pragma unroll 1
-      for (int b = 0; b < NQ; ++b)
-        {
-          R zs[N1], zd[N1], xs[N1], ys[N1];
-          R uxs[VISC ? N1 : 1], uxd[VISC ? N1 : 1], uys[VISC ? N1 : 1], uyd[VISC ? N1 : 1];
-          R WSX[N1], WDX[N1], WSY[N1], WDY[N1];
-#pragma unroll
-          for (int i = 0; i < N1; ++i)
-            {
-              R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-#pragma unroll
-              for (int j = 0; j < N1; ++j)
-                {
-                  const R zv = Z[i + N1 * j];
-                  s0 += T.Sq[b][j] * zv;
-                  s1 += T.Dq[b][j] * zv;
-                  s2 += T.Sq[b][j] * QX[i + N1 * j];
-                  s3 += T.Sq[b][j] * QY[i + N1 * j];
-                }
-              zs[i] = s0;
-              zd[i] = s1;
-              xs[i] = s2;
-              ys[i] = s3;
-              if (VISC)
-                {
-                  R t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-#pragma unroll
-                  for (int j = 0; j < N1; ++j)
-                    {
-                      t0 += T.Sq[b][j] * UX[i + N1 * j];
-                      t1 += T.Dq[b][j] * UX[i + N1 * j];
-                      t2 += T.Sq[b][j] * UY[i + N1 * j];
-                      t3 += T.Dq[b][j] * UY[i + N1 * j];
-                    }
-                  uxs[i] = t0;
-                  uxd[i] = t1;
-                  uys[i] = t2;
-                  uyd[i] = t3;
-                }
-              WSX[i] = WDX[i] = WSY[i] = WDY[i] = 0;
-            }
-#pragma unroll
-          for (int a = 0; a < NQ; ++a)
-            {
-              R z = 0, zx = 0, zy = 0, qx = 0, qy = 0;
-#pragma unroll
-              for (int i = 0; i < N1; ++i)
-                {
-                  z += T.Sq[a][i] * zs[i];
-                  zx += T.Dq[a][i] * zs[i];
-                  zy += T.Sq[a][i] * zd[i];
-                  qx += T.Sq[a][i] * xs[i];
-                  qy += T.Sq[a][i] * ys[i];
-                }
-              z += zcr;
-              zx *= ihx;
-              zy *= ihy;
-              const int q = a + NQ * b;
-              const size_t pi = c0base + (size_t)q * CB;
-              const R zb = fld(bed, pi);
-              const R h = M.depth(z, zb);
-              const R s = M.inv_depth(h);
-              const R vx = qx * s, vy = qy * s;
-              R F00 = qx * vx, F01 = qx * vy, F10 = qy * vx, F11 = qy * vy;
-              if (VISC)
-                {
-                  R G00 = 0, G01 = 0, G10 = 0, G11 = 0;
-#pragma unroll
-                  for (int i = 0; i < N1; ++i)
-                    {
-                      G00 += T.Dq[a][i] * uxs[i];
-                      G01 += T.Sq[a][i] * uxd[i];
-                      G10 += T.Dq[a][i] * uys[i];
-                      G11 += T.Sq[a][i] * uyd[i];
-                    }
-                  G00 = h * (G00 * ihx);
-                  G01 = h * (G01 * ihy);
-                  G10 = h * (G10 * ihx);
-                  G11 = h * (G11 * ihy);
-                  const R div = G00 + G11;
-                  F00 = F00 - nu * (G00 + G00) + nu * div;
-                  F01 = F01 - nu * (G01 + G10);
-                  F10 = F10 - nu * (G10 + G01);
-                  F11 = F11 - nu * (G11 + G11) + nu * div;
-                }
-              const R mgh = -g * h;
-              R Sx = mgh * zx, Sy = mgh * zy;
-              if (O.has_friction)
-                {
-                  const R n_m = R(fld(O.cell0[1], pi));
-                  const R inv_d = R(1) / (exp(log(h) * R(0.33333333333)));
-                  const R hcf = g * n_m * n_m * inv_d * sqrt(vx * vx + vy * vy);
-                  Sx -= hcf * vx;
-                  Sy -= hcf * vy;
-                }
-              if (O.has_wind)
-                {
-                  Sx += R(fld(O.cell0[2], pi));
-                  Sy += R(fld(O.cell0[3], pi));
-                }
-              if (O.has_coriolis)
-                {
-                  const R fc = R(fld(O.cell0[4], pi));
-                  Sx += fc * qy;
-                  Sy += -fc * qx;
-                }
-              const R JxW = T.wq[a] * T.wq[b] * area;
-              const R g0x = F00 * ihx * JxW, g0y = F01 * ihy * JxW;
-              const R g1x = F10 * ihx * JxW, g1y = F11 * ihy * JxW;
-              const R v0 = Sx * JxW, v1 = Sy * JxW;
-#pragma unroll
-              for (int i = 0; i < N1; ++i)
-                {
-                  WSX[i] += T.Sq[a][i] * v0 + T.Dq[a][i] * g0x;
-                  WDX[i] += T.Sq[a][i] * g0y;
-                  WSY[i] += T.Sq[a][i] * v1 + T.Dq[a][i] * g1x;
-                  WDY[i] += T.Sq[a][i] * g1y;
-                }
-            }
-#pragma unroll
-          for (int j = 0; j < N1; ++j)
-#pragma unroll
-            for (int i = 0; i < N1; ++i)
-              {
-                Racc[0][i + N1 * j] += T.Sq[b][j] * WSX[i] + T.Dq[b][j] * WDX[i];
-                Racc[1][i + N1 * j] += T.Sq[b][j] * WSY[i] + T.Dq[b][j] * WDY[i];
-              }
-        }
+      discharge_cell_term<P, VISC, R>(O, T, lds, batch, lane, hx, hy, R(zc), Racc[0], Racc[1]);
       phase_barrier();
-      face_terms<P, 2, R>(
-        O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc,
-        [&](bool is_m, R zo, Vec2<R> qo, R zn, Vec2<R> qn, Vec2<R> n, R zbm, R zbp, R *val) {
-          // minus side: -(F(own, nbr) + P(own, nbr; zbm, zbp)); plus side: F(nbr, own) +
-          // P(own, nbr; zbp, zbm), written with selects (see the height flux)
-          const Vec2<R> F = M.adv_flux(is_m ? zo : zn, is_m ? zn : zo, is_m ? qo : qn,
-                                       is_m ? qn : qo, n, zbm, zbp);
-          const Vec2<R> Pr = M.pres_flux(zo, zn, n, is_m ? zbm : zbp, is_m ? zbp : zbm);
-          val[0] = is_m ? -F.x - Pr.x : F.x + Pr.x;
-          val[1] = is_m ? -F.y - Pr.y : F.y + Pr.y;
-        },
-        [&](int bk, R zo, Vec2<R> qo, Vec2<R> n, R zb, const double *bv, R *val) {
-          R zp;
-          Vec2<R> qp;
-          const bool at_outflow = boundary_state(M, bk, zo, qo, n, zb, bv, zp, qp);
-          Vec2<R> F = M.adv_flux(zo, zp, qo, qp, n, zb, zb);
-          const Vec2<R> Pr = M.pres_flux(zo, zp, n, zb, zb);
-          F.x += Pr.x;
-          F.y += Pr.y;
-          if (at_outflow && dot_n(qo, n) < R(-1e-12))
-            F = {R(0), R(0)};
-          val[0] = -F.x;
-          val[1] = -F.y;
-        });
+      const MomentumFaces<R> fn{M};
+      face_terms<P, 2, R>(O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, Racc, fn, fn);
     }
   if (MODE == 2 || MODE == 4)
     {
@@ -1545,9 +1751,9 @@ __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 
 // L_zeta, front and depth solve of one batch of cells in one pass: L_zeta stays in
 // registers for the front (it is also written out as ki_h[stage+1] for later stages)
-template <int P, typename R>
+template <int P, typename R, bool PAIR = false>
 __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageArgs &S,
-                                                   int batch, float *lds)
+                                                   int batch, float *lds, R *xch = nullptr)
 {
   constexpr int ND = (P + 1) * (P + 1);
   const DegreeTables &T = basis<double>(P);
@@ -1556,18 +1762,50 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
   const Phys<double> M = physics<double>(O);
   int k = 0;
   double F[ND], Z[ND], area = 1.0;
-  if (active)
+  R Racc[1][ND];
+  if constexpr (PAIR)
+    {
+      // faces 0 and 1 for both equations; the momentum integrals go to the discharge wave,
+      // the mass integrals of faces 2 and 3 come from it
+      R A[3][ND];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          A[c][i] = 0;
+      if (active)
+        {
+          const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
+          const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
+          area = hx * hy;
+          R Zo[ND], QX[ND], QY[ND];
+          const double zc = load_own<P, R>(S.src_h, S.src_q, batch, lane, Zo, QX, QY);
+          height_cell_term<P, R>(basis<R>(P), R(hx), R(hy), (const R *)QX, (const R *)QY, A[0]);
+          phase_barrier();
+          const AllFaces<R> fn{physics<R>(O)};
+          face_terms<P, 3, R, 0x3>(O, basis<R>(P), batch, lane, R(hx), R(hy), (const R *)Zo,
+                                   (const R *)QX, (const R *)QY, zc, S.src_h, S.src_q, A, fn,
+                                   fn);
+        }
+      pair_exchange<P, R, 0>(A, xch);
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        Racc[0][i] = A[0][i];
+    }
+  else if (active)
     {
       const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
       const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
       area = hx * hy;
+      R Zo[ND], QX[ND], QY[ND];
+      const double zc = load_own<P, R>(S.src_h, S.src_q, batch, lane, Zo, QX, QY);
+      height_operator<P, R>(O, basis<R>(P), physics<R>(O), batch, lane, R(hx), R(hy),
+                            (const R *)Zo, (const R *)QX, (const R *)QY, zc, S.src_h,
+                            S.src_q, Racc);
+    }
+  if (active)
+    {
       {
-        R Zo[ND], QX[ND], QY[ND];
-        const double zc = load_own<P, R>(S.src_h, S.src_q, batch, lane, Zo, QX, QY);
-        R Racc[1][ND];
-        height_operator<P, R>(O, basis<R>(P), physics<R>(O), batch, lane, R(hx), R(hy),
-                              (const R *)Zo, (const R *)QX, (const R *)QY, zc, S.src_h,
-                              S.src_q, Racc);
         // front sum_j a_j k_{j+1} in the reference's order, the earlier k vectors read one
         // whole vector at a time (loads in flight together)
         const size_t base = (size_t)batch * ND * CB + lane;
@@ -1680,6 +1918,32 @@ __global__ __launch_bounds__(CB * WPB, (stage_min_waves<P, VISC, R>())) void k_s
     discharge_block<P, VISC, 0, R>(O, Q, w >> 1, lds[wave]);
   else
     height_stage_block<P, R>(O, H, w >> 1, reinterpret_cast<float *>(lds[wave]));
+}
+
+// LDS offset (words of R) of the paired items' exchange area inside the height wave's slice:
+// past the words the FP32 depth increment's matrix-core product uses
+template <int P, typename R>
+__host__ __device__ constexpr int pair_xch_offset()
+{
+  return std::is_same<R, float>::value && P >= 3 ? matvec_lds_words<(P + 1) * (P + 1), 1>() : 0;
+}
+
+// the whole stage in one launch with paired work items: workgroup = one batch, wave 0 its
+// height item and wave 1 its discharge item, the faces split between them (pair_exchange);
+// workgroups are dealt so that each XCD holds a contiguous range of batches
+template <int P, typename R>
+__global__ __launch_bounds__(2 * CB, (stage_min_waves<P, false, R>())) void k_stage_pair(
+  OpArgs O, StageArgs H, StageArgs Q)
+{
+  __shared__ R lds[2][discharge_lds_words<P, false>()];
+  static_assert(pair_xch_offset<P, R>() + pair_xch_words<P>() <= discharge_lds_words<P, false>(),
+                "LDS of the height item too small for the paired exchange");
+  const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
+  R *xch = lds[0] + pair_xch_offset<P, R>();
+  if (threadIdx.x < CB)
+    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch);
+  else
+    discharge_block<P, false, 0, R, true>(O, Q, batch, lds[1], xch);
 }
 
 }  // namespace oceano

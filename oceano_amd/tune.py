@@ -7,8 +7,12 @@ The tunable parameters are the library options read at operator creation:
   xcd_swizzle      each XCD runs a contiguous batch range (1) or the round-robin order (0)
   waves_per_block  64-cell work items per workgroup of the one-pass stage kernel (1, 2, 4):
                    64, 128 or 256 cells per workgroup, the LDS of each wave its own slice
-Results do not depend on them (each cell's arithmetic is the same); their speed depends on
-the degree, the mesh and the precision.  `autotune` times graph-replayed RK4 steps of the
+  paired_items     a batch's height and discharge items in one two-wave workgroup that
+                   splits the faces between them (1), or independent items (0); -1 picks
+                   by precision (paired for mixed)
+Results do not depend on the first three (each cell's arithmetic is the same) and depend
+on paired_items only at rounding level on hanging faces (another summation order); their
+speed depends on the degree, the mesh and the precision.  `autotune` times graph-replayed RK4 steps of the
 problem for each candidate and returns the fastest setting; `apply_options` sets it for
 the operators created next.
 """
@@ -17,15 +21,18 @@ import time
 from . import _capi
 
 DEFAULT_CANDIDATES = [
-    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 1},
-    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 1},
-    {"slot_chunk": -1, "xcd_swizzle": 0, "waves_per_block": 1},
-    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 2},
-    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 4},
-    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 2},
-    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 4},
+    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 1, "paired_items": 0},
+    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 1, "paired_items": 0},
+    {"slot_chunk": -1, "xcd_swizzle": 0, "waves_per_block": 1, "paired_items": 0},
+    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 2, "paired_items": 0},
+    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 4, "paired_items": 0},
+    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 2, "paired_items": 0},
+    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 4, "paired_items": 0},
+    {"slot_chunk": -1, "xcd_swizzle": 1, "waves_per_block": 1, "paired_items": 1},
+    {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 1, "paired_items": 1},
 ]
-LIBRARY_DEFAULTS = {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 1}
+LIBRARY_DEFAULTS = {"slot_chunk": 0, "xcd_swizzle": 1, "waves_per_block": 1,
+                    "paired_items": -1}
 
 
 def apply_options(opts):

@@ -145,14 +145,17 @@ def test_waves_per_block_does_not_change_results(cuda, wpb):
     from oceano_amd import tune
     P = problems.vortex(degree=3, refine=1, refine_fraction=0.3, seed=5)
     out = []
-    for w in (1, wpb):
-        tune.apply_options({"waves_per_block": w})
-        for precision in ("fp64", "mixed"):
-            op, _, h, q = problems.setup_operator(P, precision=precision)
-            rk = DeviceIntegrator(op)
-            for _ in range(2):
-                rk.step(1e-4, h, q)
-            out.append((h, q))
-    tune.apply_options({"waves_per_block": 1})
+    try:
+        for w in (1, wpb):
+            # unpaired items: with paired_items a workgroup is always one batch's two items
+            tune.apply_options({"waves_per_block": w, "paired_items": 0})
+            for precision in ("fp64", "mixed"):
+                op, _, h, q = problems.setup_operator(P, precision=precision)
+                rk = DeviceIntegrator(op)
+                for _ in range(2):
+                    rk.step(1e-4, h, q)
+                out.append((h, q))
+    finally:
+        tune.apply_options(tune.LIBRARY_DEFAULTS)
     for a, b in zip(out[:2], out[2:]):
         assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])

@@ -1,6 +1,8 @@
-"""Work-layout options (cell-slot order, batch-to-XCD mapping) change only the speed: an RK4
-step under every option set the autotuner tries is bitwise the same, in both precisions,
-and the autotuner returns one of its candidates with a timing for each.
+"""Work-layout options (cell-slot order, batch-to-XCD mapping, work items per workgroup)
+change only the speed: an RK4 step under every option set the autotuner tries is bitwise the
+same, in both precisions, among the candidates with the same paired_items setting (pairing
+sums the face integrals of hanging faces in another order: tests/test_gpu_paired.py), and
+the autotuner returns one of its candidates with a timing for each.
 
 The slot orders also change how the FP32 stage deals the second subfaces of coarse faces
 over the lanes of a wave (one item per lane, several passes when a wave holds more than 64
@@ -37,8 +39,10 @@ def _max_second_subfaces_per_wave(op, mesh):
 def test_options_do_not_change_results(cuda, precision):
     P = problems.vortex(degree=3, refine=1, refine_fraction=0.3, seed=12)
     print("[tune] %s, %d cells" % (precision, P.mesh.n_cells), flush=True)
-    ref = None
-    cands = tune.DEFAULT_CANDIDATES + [{"slot_chunk": 4, "xcd_swizzle": 1}]
+    ref = {}
+    cands = tune.DEFAULT_CANDIDATES + [
+        {"slot_chunk": 4, "xcd_swizzle": 1, "waves_per_block": 1, "paired_items": 0},
+        {"slot_chunk": 4, "xcd_swizzle": 1, "waves_per_block": 1, "paired_items": 1}]
     most = 0
     try:
         for opts in cands:
@@ -50,10 +54,8 @@ def test_options_do_not_change_results(cuda, precision):
                 rk.step(1e-4, w_h, w_q)
             torch.cuda.synchronize()
             out = (to_host(op, w_h, 1), to_host(op, w_q, 2))
-            if ref is None:
-                ref = out
-            else:
-                assert (out[0] == ref[0]).all() and (out[1] == ref[1]).all(), opts
+            r = ref.setdefault(opts["paired_items"], out)
+            assert (out[0] == r[0]).all() and (out[1] == r[1]).all(), opts
     finally:
         tune.apply_options(tune.LIBRARY_DEFAULTS)
     print("  most second subfaces in one wave: %d" % most)
