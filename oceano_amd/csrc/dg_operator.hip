@@ -482,9 +482,11 @@ bool uniform(const std::vector<double> &v, double &val)
 int g_xcd_swizzle = 1;  // batch -> workgroup order: each XCD gets a contiguous batch range
 int g_waves_per_block = 1;  // work items (waves of 64 cells) per workgroup of the stage
 // inviscid one-pass stage: a batch's two items in one workgroup, faces split between them;
-// < 0: by precision -- the mixed path pairs (measured on the C2 vortex: 58.2 -> 52.8 us per
-// stage), FP64 does not (114.4 -> 119.9 us: at one wave per SIMD the pair's barrier costs
-// more than the shared face work saves)
+// < 0: by precision and degree -- paired where the kernel runs 3 or more waves per SIMD,
+// and for FP64 at p = 2 (measured: mixed p = 3 on the hanging C2 vortex 58.2 -> 52.8 us per
+// stage; the ensemble's groups, paired / unpaired step time, FP64 0.88, 0.96, 1.19, 1.22 and
+// mixed 0.85, 0.89, 0.97, 1.11 for p = 1..4: at one or two waves per SIMD the pair's barrier
+// costs more than the shared face work saves)
 int g_paired_items = -1;
 int g_slot_chunk = 0;  // batches per chunk; 0: by precision as above; < 0: the caller's order
 
@@ -644,7 +646,8 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
   op->xcd_swizzle = g_xcd_swizzle;
   op->waves_per_block = g_waves_per_block;
   op->paired_items =
-    g_paired_items < 0 ? (prm->precision == OC_PRECISION_MIXED ? 1 : 0) : g_paired_items;
+    g_paired_items >= 0 ? g_paired_items
+                        : (prm->precision == OC_PRECISION_MIXED ? prm->degree <= 3 : prm->degree <= 2);
   op->P = prm->degree;
   op->n1 = op->P + 1;
   op->nd = op->n1 * op->n1;

@@ -4,7 +4,9 @@
 // The instances of an ensemble are grouped into one operator per degree (a disjoint union
 // of their meshes); this kernel advances every group in a single grid: workgroup b belongs
 // to the group whose range of workgroups holds it and runs that group's work item (batch,
-// kind) at that group's degree, exactly as the per-degree one-pass kernel k_stage does.
+// kind) at that group's degree, exactly as the per-degree one-pass kernel k_stage does
+// (k_stage_multi_pair: workgroup b runs a batch's two items paired, as k_stage_pair does,
+// when every operator pairs its items).
 // Each group's range starts at a multiple of the XCD count, so the batch -> XCD dealing of
 // the per-degree kernel holds inside every group.  The kernel's register and LDS budget is
 // that of its largest degree.  No reference counterpart: oceano runs one instance per
@@ -81,6 +83,50 @@ __global__ __launch_bounds__(CB, (multi_min_waves<R>())) void k_stage_multi(
     }
 }
 
+// paired items (k_stage_pair): workgroup = one batch of a group, wave 0 its height item,
+// wave 1 its discharge item
+template <int P, typename R>
+__device__ __forceinline__ void multi_pair(const OpArgs &O, const StageArgs &H,
+                                           const StageArgs &Q, int batch, R (&lds)[2][discharge_lds_words<4, false>()])
+{
+  R *xch = lds[0] + pair_xch_offset<P, R>();
+  if (threadIdx.x < CB)
+    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch);
+  else
+    discharge_block<P, false, 0, R, true>(O, Q, batch, lds[1], xch);
+}
+
+template <typename R>
+__global__ __launch_bounds__(2 * CB, (multi_min_waves<R>())) void k_stage_multi_pair(
+  const MultiArgs *__restrict__ A)
+{
+  __shared__ R lds[2][discharge_lds_words<4, false>()];
+  const int b = blockIdx.x;
+  int g = 0;
+  while (g + 1 < A->n_groups && b >= A->start[g + 1])
+    ++g;
+  const OpArgs &O = A->O[g];
+  const int local = b - A->start[g];
+  if (local >= O.n_batches)  // padding of the group's range to a multiple of the XCD count
+    return;
+  const int batch = logical_batch(local, O.n_batches, O.xcd_swizzle);
+  switch (A->degree[g])
+    {
+      case 1:
+        multi_pair<1, R>(O, A->H[g], A->Q[g], batch, lds);
+        break;
+      case 2:
+        multi_pair<2, R>(O, A->H[g], A->Q[g], batch, lds);
+        break;
+      case 3:
+        multi_pair<3, R>(O, A->H[g], A->Q[g], batch, lds);
+        break;
+      default:
+        multi_pair<4, R>(O, A->H[g], A->Q[g], batch, lds);
+        break;
+    }
+}
+
 }  // namespace
 
 int upload_tables_multi(const DegreeTables *t, const Basis<float> *tf)
@@ -102,6 +148,11 @@ int stage_multi(int n, oc_dg *const *ops, const OpArgs *O, uint32_t stage,
 {
   MultiArgs A{};
   A.n_groups = n;
+  // paired items when every operator asks for them (one workgroup per batch), else one
+  // workgroup per work item
+  bool paired = true;
+  for (int g = 0; g < n; ++g)
+    paired = paired && ops[g]->paired_items;
   int start = 0;
   for (int g = 0; g < n; ++g)
     {
@@ -110,14 +161,19 @@ int stage_multi(int n, oc_dg *const *ops, const OpArgs *O, uint32_t stage,
       A.O[g] = O[g];
       stage_args(O[g], stage, factor[g], ri_h[g], ri_q[g], nullptr, ki_h[g], ki_q[g], n_ki,
                  w_h[g], w_q[g], r_h[g], r_q[g], 0, A.H[g], A.Q[g]);
-      start += (2 * ops[g]->NB + NXCD - 1) / NXCD * NXCD;
+      const int blocks = paired ? ops[g]->NB : 2 * ops[g]->NB;
+      start += (blocks + NXCD - 1) / NXCD * NXCD;
     }
   A.start[n] = start;
   const bool mixed = ops[0]->precision == OC_PRECISION_MIXED;
   MultiArgs *dA = ops[0]->d_multi_args;
   hipLaunchKernelGGL(k_put_multi_args, dim3(1), dim3(CB), 0, s, A, dA);
   HIP_TRY(hipGetLastError());
-  if (mixed)
+  if (paired && mixed)
+    hipLaunchKernelGGL(k_stage_multi_pair<float>, dim3(start), dim3(2 * CB), 0, s, dA);
+  else if (paired)
+    hipLaunchKernelGGL(k_stage_multi_pair<double>, dim3(start), dim3(2 * CB), 0, s, dA);
+  else if (mixed)
     hipLaunchKernelGGL(k_stage_multi<float>, dim3(start), dim3(CB), 0, s, dA);
   else
     hipLaunchKernelGGL(k_stage_multi<double>, dim3(start), dim3(CB), 0, s, dA);
