@@ -43,8 +43,7 @@ const char *oc_last_error(void);
  *   "waves_per_block" 1 (default), 2 or 4: 64-cell work items per workgroup of the one-pass
  *                 stage kernel (64, 128 or 256 cells per workgroup).  Results do not
  *                 depend on it.
- *   "paired_items" -1 (default: 1 for OC_PRECISION_MIXED at degree <= 3 and for FP64 at
- *                 degree <= 2, else 0), 0 or 1: the
+ *   "paired_items" -1 (default: 1 at degree <= 3, 0 at degree 4), 0 or 1: the
  *                 inviscid one-pass stage runs each batch's height and discharge items as
  *                 one two-wave workgroup that splits the faces between them (each face
  *                 point's fluxes formed once per cell, the other equation's integrals

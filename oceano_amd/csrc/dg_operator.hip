@@ -482,11 +482,11 @@ bool uniform(const std::vector<double> &v, double &val)
 int g_xcd_swizzle = 1;  // batch -> workgroup order: each XCD gets a contiguous batch range
 int g_waves_per_block = 1;  // work items (waves of 64 cells) per workgroup of the stage
 // inviscid one-pass stage: a batch's two items in one workgroup, faces split between them;
-// < 0: by precision and degree -- paired where the kernel runs 3 or more waves per SIMD,
-// and for FP64 at p = 2 (measured: mixed p = 3 on the hanging C2 vortex 58.2 -> 52.8 us per
-// stage; the ensemble's groups, paired / unpaired step time, FP64 0.88, 0.96, 1.19, 1.22 and
-// mixed 0.85, 0.89, 0.97, 1.11 for p = 1..4: at one or two waves per SIMD the pair's barrier
-// costs more than the shared face work saves)
+// < 0: by degree -- paired at p <= 3 (measured, paired /
+// unpaired time: hanging 97k-cell vortex at p = 3, mixed 52.0 / 57.7 us, FP64 100.9 / 104.9
+// us per stage; the conforming ensemble groups' steps, FP64 0.87, 0.96, 1.08, 1.24 and mixed
+// 0.86, 0.90, 0.96, 1.13 for p = 1..4: at p = 4 (one or two waves per SIMD) the pair's
+// barrier costs more than the shared face work saves)
 int g_paired_items = -1;
 int g_slot_chunk = 0;  // batches per chunk; 0: by precision as above; < 0: the caller's order
 
@@ -645,9 +645,7 @@ int oc_dg_create(const oc_mesh_desc *mesh, const oc_params *prm, oc_dg **out)
   oc_dg *op = new oc_dg;
   op->xcd_swizzle = g_xcd_swizzle;
   op->waves_per_block = g_waves_per_block;
-  op->paired_items =
-    g_paired_items >= 0 ? g_paired_items
-                        : (prm->precision == OC_PRECISION_MIXED ? prm->degree <= 3 : prm->degree <= 2);
+  op->paired_items = g_paired_items >= 0 ? g_paired_items : prm->degree <= 3;
   op->P = prm->degree;
   op->n1 = op->P + 1;
   op->nd = op->n1 * op->n1;

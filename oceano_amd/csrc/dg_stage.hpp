@@ -897,6 +897,10 @@ __device__ __forceinline__ void discharge_cell_term(const OpArgs &O, const Basis
 #pragma unroll 1
   for (int b = 0; b < NQ; ++b)
     {
+      // the own nodal values are read from LDS again in every row: hoisted out of the loop
+      // they would stay live in 3 nd registers (FP64: 6 nd) across the whole cell term,
+      // which held the FP64 kernels at p = 3 to one wave per SIMD
+      asm volatile("" ::: "memory");
       R zs[N1], zd[N1], xs[N1], ys[N1];
       R uxs[VISC ? N1 : 1], uxd[VISC ? N1 : 1], uys[VISC ? N1 : 1], uyd[VISC ? N1 : 1];
       R WSX[N1], WDX[N1], WSY[N1], WDY[N1];
@@ -1887,7 +1891,7 @@ template <int P, bool VISC, typename R>
 constexpr int stage_min_waves()
 {
   return VISC ? 1 : std::is_same<R, float>::value ? (P == 1 ? 5 : P == 2 ? 4 : P == 3 ? 3 : 2)
-                                                  : (P == 1 ? 3 : P == 2 ? 2 : 1);
+                                                  : (P == 1 ? 3 : P == 2 ? 2 : P == 3 ? 2 : 1);
 }
 
 // Kind of work item w (0 = height stage, 1 = discharge stage) of batch w / 2: the parity of

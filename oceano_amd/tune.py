@@ -9,7 +9,7 @@ The tunable parameters are the library options read at operator creation:
                    64, 128 or 256 cells per workgroup, the LDS of each wave its own slice
   paired_items     a batch's height and discharge items in one two-wave workgroup that
                    splits the faces between them (1), or independent items (0); -1 picks
-                   by precision and degree (paired for mixed p <= 3 and FP64 p <= 2)
+                   by degree (paired for p <= 3)
 Results do not depend on the first three (each cell's arithmetic is the same) and depend
 on paired_items only at rounding level on hanging faces (another summation order); their
 speed depends on the degree, the mesh and the precision.  `autotune` times graph-replayed RK4 steps of the
