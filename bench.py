@@ -405,6 +405,7 @@ def main():
         eager = time.perf_counter() - t1
         gc.enable()
         kern = {}
+        per_op = []
         for g in G:
             op = g["op"]
             rep = op.timing_report()
@@ -419,7 +420,14 @@ def main():
                 e = kern.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0.0})
                 e["ms"] += ms
                 e["launches"] += n
-                e["bytes"] += sum(per_stage[j % S][name] for j in range(n_steps_timed))
+                b = sum(per_stage[j % S][name] for j in range(n_steps_timed))
+                e["bytes"] += b
+                if name == "stage":  # each operator's own stage kernel (per degree)
+                    gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
+                    per_op.append({"degree": g["P"].degree, "cells": g["P"].mesh.n_cells,
+                                   "stage_us": ms / n * 1e3, "alg_bytes": b / n,
+                                   "alg_GBs": gbs,
+                                   "frac": gbs / HBM_PEAK_GBS if gbs else None})
         torch.cuda.synchronize()
         kernels = {}
         for name, e in kern.items():
@@ -429,6 +437,7 @@ def main():
             kernels[name] = {"mean_ms": mean_ms, "launches": e["launches"], "alg_bytes": b,
                              "alg_GBs": b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None}
         out = {"elapsed": elapsed, "wall": wall, "eager": eager, "kernels": kernels,
+               "per_operator": per_op,
                "groups": G,
                "nonfinite": sum(g["op"].count_nonfinite(g["w_h"]) +
                                 g["op"].count_nonfinite(g["w_q"]) for g in G),
@@ -500,6 +509,8 @@ def main():
                    "eager_value": dofs * n_steps_timed / r["eager"], "kernels": r["kernels"],
                    "nonfinite": r["nonfinite"],
                    "jacobi_sweeps_last_stage": r["sweeps_last_stage"]}
+        if multi:  # each degree group's stage kernel against the HBM roofline
+            line[p]["per_operator"] = r["per_operator"]
         if "errors" in r:
             line[p]["l2_error_vs_exact"] = r["errors"]
             line[p]["t_end"] = r["t_end"] if multi else r["t_end"][0]
