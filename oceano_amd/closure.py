@@ -567,6 +567,71 @@ def evaluate_heldout(net, n_eval=32, T=2.0, n_snap=4, eval_seed0=2000, log=None)
                                 for k, v in err.items()} if "sampled" in err else None}
 
 
+def frozen_diagnosis(net, seeds, T=2.0, n_snap=4, log=None):
+    """Which part of the learned correction harms the coupled run: coarse runs with frozen
+    momentum sources (each correction evaluated once at the initial state and held fixed
+    through the wind-stress slot, _frozen_source) built from the exact and the learned
+    correction (at gain 1) and mixtures of their modes.  The cell-mean modes are the first
+    mode of each momentum component.  Returns {seed: {"errors": {case: (zeta, hu)},
+    "modes": statistics of the cell-mean modes on the active cells}}."""
+    import torch
+    g0 = net.output_gain
+    net.set_output_gain(1.0)
+    out = {}
+    for seed in seeds:
+        s = Sample(seed, T=T, n_snap=n_snap)
+        S = s.S
+        n = S.plain.mesh.n_cells
+        nd = (S.degree + 1) ** 2
+        act = np.asarray(s.active, bool)
+        Te = s.T_all[:n].double().cpu().numpy()
+        Tl = net.forward(s.X_all[:n].contiguous()).double().cpu().numpy()
+        Tl[~act] = 0.0
+        means = [0, nd]
+        unb = Tl.copy()
+        for j in means:
+            unb[act, j] -= Tl[act, j].mean()
+        cases = {"plain": None, "exact": Te, "learned": Tl}
+        mix = Tl.copy()
+        mix[:, means] = Te[:, means]
+        cases["learned, exact cell means"] = mix
+        mix = Te.copy()
+        mix[:, means] = Tl[:, means]
+        cases["exact, learned cell means"] = mix
+        cases["learned, net force removed"] = unb
+        cases["learned x 0.25, net force removed"] = 0.25 * unb
+        t0, rh0, rq0 = s.states[0]
+        tT, rhT, rqT = s.states[-1]
+        hx, hy = S.length / S.nx, 1.0 / S.ny
+        _, V = nodal_to_modal(S.degree)
+        err = {}
+        for name, tau in cases.items():
+            op, data, _, _ = problems.setup_operator(S.plain)
+            if tau is not None:
+                _frozen_source(op, data, S.plain.mesh,
+                               np.stack([tau[:, :nd] @ V.T, tau[:, nd:] @ V.T], 1))
+            h, q = op.pack(rh0, 1), op.pack(rq0, 2)
+            dt, nst = _cfl_dt(op, h, q, s.cfl, tT - t0)
+            _integrate(op, dt, nst, h, q)
+            eh = l2_norms(op.unpack(h, 1) - rhT, 1, S.degree, hx, hy)
+            eq = l2_norms(op.unpack(q, 2) - rqT, 2, S.degree, hx, hy)
+            err[name] = (float(eh[0]), float(eq[0]))
+        d = (Tl - Te)[act]
+        modes = {"exact_rms": float(np.sqrt((Te[act][:, means] ** 2).mean())),
+                 "learned_error_rms": float(np.sqrt((d[:, means] ** 2).mean())),
+                 "learned_minus_exact_hu_mean_average": float(d[:, 0].mean()),
+                 "exact_hu_mean_average": float(Te[act][:, 0].mean())}
+        out[seed] = {"errors": err, "modes": modes}
+        if log:
+            pz, pq = err["plain"]
+            log("seed %d: " % seed + "; ".join("%s gain %.2f %.2f" % (k, pz / v[0], pq / v[1])
+                                               for k, v in err.items()) + "; %s" % modes)
+        del s
+        torch.cuda.synchronize()
+    net.set_output_gain(g0)
+    return out
+
+
 def main(argv=None):
     import argparse
     import json
@@ -580,11 +645,23 @@ def main(argv=None):
                     help="a trained network (.npz): calibrate its output gain on training "
                          "bathymetries and evaluate it on the held-out ones (no training)")
     ap.add_argument("--cal-seeds", type=int, default=8)
+    ap.add_argument("--diagnose", type=int, nargs="*", default=None,
+                    help="held-out seeds: frozen-source diagnosis of the shipped network "
+                         "(frozen_diagnosis)")
     ap.add_argument("--out", default="closure_run")
     a = ap.parse_args(argv)
     import os
     os.makedirs(a.out, exist_ok=True)
     log = lambda m: print(m, flush=True)  # noqa: E731
+    if a.diagnose is not None:
+        from .mlp import ClosureNetwork
+        net = ClosureNetwork.load(os.path.join(os.path.dirname(__file__), "data",
+                                               "closure_channel_p2.npz"))
+        rep = frozen_diagnosis(net, a.diagnose or [2000, 2001, 2002, 2003], T=a.time, log=log)
+        with open(os.path.join(a.out, "diagnosis.json"), "w") as f:
+            json.dump({str(k): v for k, v in rep.items()}, f, indent=1)
+        print(json.dumps({str(k): v for k, v in rep.items()}))
+        return
     if a.calibrate:
         import time
         from .mlp import ClosureNetwork
