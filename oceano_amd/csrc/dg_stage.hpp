@@ -1095,8 +1095,11 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   R area = 1;
   if constexpr (PAIR)
     {
-      // faces 2 and 3 for both equations (pair_exchange)
+      // faces 2 and 3 for both equations (pair_exchange); the LDS copy of the batch's own
+      // data staged above is the height wave's too (read after this barrier, until the
+      // exchange)
       static_assert(!VISC && MODE == 0, "paired items: inviscid explicit stage only");
+      __syncthreads();
       R A[3][ND];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -1757,7 +1760,8 @@ __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 // registers for the front (it is also written out as ki_h[stage+1] for later stages)
 template <int P, typename R, bool PAIR = false>
 __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageArgs &S,
-                                                   int batch, float *lds, R *xch = nullptr)
+                                                   int batch, float *lds, R *xch = nullptr,
+                                                   const R *own = nullptr)
 {
   constexpr int ND = (P + 1) * (P + 1);
   const DegreeTables &T = basis<double>(P);
@@ -1770,26 +1774,27 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
   if constexpr (PAIR)
     {
       // faces 0 and 1 for both equations; the momentum integrals go to the discharge wave,
-      // the mass integrals of faces 2 and 3 come from it
+      // the mass integrals of faces 2 and 3 come from it.  The batch's own data are read
+      // from the discharge wave's LDS copy (`own`, staged before the first barrier)
       R A[3][ND];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int i = 0; i < ND; ++i)
           A[c][i] = 0;
+      const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
+      const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
+      const double zc = centred<R>() ? S.src_h[vix(batch, 1, 0, ND, 0, lane)] : 0.0;
+      __syncthreads();
       if (active)
         {
-          const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
-          const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
           area = hx * hy;
-          R Zo[ND], QX[ND], QY[ND];
-          const double zc = load_own<P, R>(S.src_h, S.src_q, batch, lane, Zo, QX, QY);
-          height_cell_term<P, R>(basis<R>(P), R(hx), R(hy), (const R *)QX, (const R *)QY, A[0]);
+          const LdsVec<R> Zo{own + lane}, QX{own + ND * CB + lane}, QY{own + 2 * ND * CB + lane};
+          height_cell_term<P, R>(basis<R>(P), R(hx), R(hy), QX, QY, A[0]);
           phase_barrier();
           const AllFaces<R> fn{physics<R>(O)};
-          face_terms<P, 3, R, 0x3>(O, basis<R>(P), batch, lane, R(hx), R(hy), (const R *)Zo,
-                                   (const R *)QX, (const R *)QY, zc, S.src_h, S.src_q, A, fn,
-                                   fn);
+          face_terms<P, 3, R, 0x3>(O, basis<R>(P), batch, lane, R(hx), R(hy), Zo, QX, QY, zc,
+                                   S.src_h, S.src_q, A, fn, fn);
         }
       pair_exchange<P, R, 0>(A, xch);
 #pragma unroll
@@ -1945,7 +1950,8 @@ __global__ __launch_bounds__(2 * CB, (stage_min_waves<P, false, R>())) void k_st
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
   R *xch = lds[0] + pair_xch_offset<P, R>();
   if (threadIdx.x < CB)
-    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch);
+    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch,
+                                   lds[1]);
   else
     discharge_block<P, false, 0, R, true>(O, Q, batch, lds[1], xch);
 }

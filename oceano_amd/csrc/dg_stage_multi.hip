@@ -91,7 +91,8 @@ __device__ __forceinline__ void multi_pair(const OpArgs &O, const StageArgs &H,
 {
   R *xch = lds[0] + pair_xch_offset<P, R>();
   if (threadIdx.x < CB)
-    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch);
+    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch,
+                                   lds[1]);
   else
     discharge_block<P, false, 0, R, true>(O, Q, batch, lds[1], xch);
 }
