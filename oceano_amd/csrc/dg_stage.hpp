@@ -1938,8 +1938,10 @@ __host__ __device__ constexpr int pair_xch_offset()
 }
 
 // the whole stage in one launch with paired work items: workgroup = one batch, wave 0 its
-// height item and wave 1 its discharge item, the faces split between them (pair_exchange);
-// workgroups are dealt so that each XCD holds a contiguous range of batches
+// height item and wave 1 its discharge item, the faces split between them (pair_exchange).
+// Two workgroup barriers: after the discharge wave has staged the batch's own data in its
+// LDS slice (which the height wave then reads too), and in pair_exchange.  Workgroups are
+// dealt so that each XCD holds a contiguous range of batches.
 template <int P, typename R>
 __global__ __launch_bounds__(2 * CB, (stage_min_waves<P, false, R>())) void k_stage_pair(
   OpArgs O, StageArgs H, StageArgs Q)
