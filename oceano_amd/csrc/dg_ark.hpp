@@ -34,26 +34,16 @@ struct ArkArgs
   double *wq_out;      // w_q at the last stage
 };
 
-template <int P, typename R>
-__global__ __launch_bounds__(CB) void k_stiff(OpArgs O, const double *__restrict__ ri_h,
-                                              const double *__restrict__ ri_q,
-                                              double *__restrict__ dst_buf)
+// the stiff friction source of one cell tested against the basis: Rx, Ry (Gauss(nq)); own
+// data Z (deviation from zcr), QX, QY through accessors (registers or an LDS copy)
+template <int P, typename R, typename ZA, typename QA>
+__device__ __forceinline__ void stiff_source(const OpArgs &O, const Basis<R> &T, const ZA &Z,
+                                             const QA &QX, const QA &QY, R zcr, int batch,
+                                             int lane, R area, R (&Rx)[(P + 1) * (P + 1)],
+                                             R (&Ry)[(P + 1) * (P + 1)])
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P);
-  const Basis<R> &T = basis<R>(P);
-  R *const dst = reinterpret_cast<R *>(dst_buf);  // the stiff k vector, stored in R
-  const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
-  const int lane = threadIdx.x;
-  if (batch * CB + lane >= O.n_cells)
-    return;
   const Phys<R> M = physics<R>(O);
-  const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
-  const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
-  const R area = hx * hy;
-  R Z[ND], QX[ND], QY[ND];
-  const double zc = load_own<P, R>(ri_h, ri_q, batch, lane, Z, QX, QY);
-  const R zcr = R(zc);
-  R Rx[ND], Ry[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k)
     Rx[k] = Ry[k] = 0;
@@ -117,6 +107,27 @@ __global__ __launch_bounds__(CB) void k_stiff(OpArgs O, const double *__restrict
             Ry[i + N1 * j] += T.Sq[b][j] * WY[i];
           }
     }
+}
+
+template <int P, typename R>
+__global__ __launch_bounds__(CB) void k_stiff(OpArgs O, const double *__restrict__ ri_h,
+                                              const double *__restrict__ ri_q,
+                                              double *__restrict__ dst_buf)
+{
+  constexpr int ND = (P + 1) * (P + 1);
+  const Basis<R> &T = basis<R>(P);
+  R *const dst = reinterpret_cast<R *>(dst_buf);  // the stiff k vector, stored in R
+  const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
+  const int lane = threadIdx.x;
+  if (batch * CB + lane >= O.n_cells)
+    return;
+  const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
+  const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
+  R Z[ND], QX[ND], QY[ND];
+  const double zc = load_own<P, R>(ri_h, ri_q, batch, lane, Z, QX, QY);
+  R Rx[ND], Ry[ND];
+  stiff_source<P, R>(O, T, (const R *)Z, (const R *)QX, (const R *)QY, R(zc), batch, lane,
+                     hx * hy, Rx, Ry);
 #pragma unroll
   for (int k = 0; k < ND; ++k)
     {
@@ -187,18 +198,16 @@ __device__ __forceinline__ void tensor_apply(const double (&A)[MAXN1][MAXN1], co
       }
 }
 
-// KT: storage type of the discharge k vectors (the integral precision)
+// the discharge update of one cell (lane) of an active batch slot; KT: storage type of the
+// discharge k vectors (the integral precision)
 template <int P, typename KT>
-__global__ __launch_bounds__(CB) void k_ark_update(OpArgs O, ArkArgs A)
+__device__ __forceinline__ void ark_update_cell(const OpArgs &O, const ArkArgs &A, int batch,
+                                                int lane)
 {
   auto kex = [&](int j, size_t i) { return double(reinterpret_cast<const KT *>(A.kex[j])[i]); };
   auto kim = [&](int j, size_t i) { return double(reinterpret_cast<const KT *>(A.kim[j])[i]); };
   constexpr int N1 = P + 1, ND = N1 * N1;
   const DegreeTables &T = basis<double>(P);
-  const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
-  const int lane = threadIdx.x;
-  if (batch * CB + lane >= O.n_cells)
-    return;
   const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
   const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
   const double area = hx * hy;
@@ -320,6 +329,78 @@ __global__ __launch_bounds__(CB) void k_ark_update(OpArgs O, ArkArgs A)
     }
 }
 
+template <int P, typename KT>
+__global__ __launch_bounds__(CB) void k_ark_update(OpArgs O, ArkArgs A)
+{
+  const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
+  const int lane = threadIdx.x;
+  if (batch * CB + lane >= O.n_cells)
+    return;
+  ark_update_cell<P, KT>(O, A, batch, lane);
+}
+
+// The momentum item of the paired additive stage: the explicit integrals (k_stage_pair's
+// discharge item in its rhs form: faces 2 and 3 and the cell, plus the exchanged faces 0
+// and 1) stored as kex; before the last stage the stiff friction source of the same stage
+// input (read from the LDS copy of the batch's own data, still intact) stored as kim; then
+// the modified-mass update of each cell (ark_update_cell, which reads this stage's kex and
+// kim back from the vectors this lane has just written).
+template <int P, typename R>
+__device__ __forceinline__ void discharge_ark_block(const OpArgs &O, const StageArgs &Q,
+                                                    const ArkArgs &A, int batch, R *lds,
+                                                    R *xch)
+{
+  constexpr int ND = (P + 1) * (P + 1);
+  discharge_block<P, false, 4, R, true>(O, Q, batch, lds, xch);
+  const int lane = threadIdx.x & (CB - 1);
+  if (batch * CB + lane >= O.n_cells)
+    return;
+  if (!A.last)
+    {
+      const R hx = R(O.hxy[((size_t)batch * 2) * CB + lane]);
+      const R hy = R(O.hxy[((size_t)batch * 2 + 1) * CB + lane]);
+      const double zc = centred<R>() ? Q.src_h[vix(batch, 1, 0, ND, 0, lane)] : 0.0;
+      const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
+      R Rx[ND], Ry[ND];
+      stiff_source<P, R>(O, basis<R>(P), Z, QX, QY, R(zc), batch, lane, hx * hy, Rx, Ry);
+      R *kim = reinterpret_cast<R *>(const_cast<double *>(A.kim[A.stage]));
+#pragma unroll
+      for (int k = 0; k < ND; ++k)
+        {
+          kim[vix(batch, 2, 0, ND, k, lane)] = Rx[k];
+          kim[vix(batch, 2, 1, ND, k, lane)] = Ry[k];
+        }
+    }
+  ark_update_cell<P, R>(O, A, batch, lane);
+}
+
+// The additive stage in one launch with paired work items (inviscid): workgroup = one batch,
+// wave 0 the height item (L_zeta, front and depth solve: the explicit stage's height item
+// with the additive stage's factors), wave 1 the momentum item (discharge_ark_block).  The
+// operator arguments carry the friction switched off before the last stage (the stiff term
+// takes it; local_apply_cell_nonstiff_discharge).
+// waves per SIMD of the paired additive stage: the momentum item's FP64 modified-mass update
+// needs more registers than the explicit stage's momentum item
+template <int P, typename R>
+constexpr int ark_min_waves()
+{
+  return std::is_same<R, float>::value ? (P <= 2 ? 3 : 2) : (P == 1 ? 2 : 1);
+}
+
+template <int P, typename R>
+__global__ __launch_bounds__(2 * CB, (ark_min_waves<P, R>())) void k_stage_ark_pair(
+  OpArgs O, StageArgs H, StageArgs Q, ArkArgs A)
+{
+  __shared__ R lds[2][discharge_lds_words<P, false>()];
+  const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
+  R *xch = lds[0] + pair_xch_offset<P, R>();
+  if (threadIdx.x < CB)
+    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch,
+                                   lds[1]);
+  else
+    discharge_ark_block<P, R>(O, Q, A, batch, lds[1], xch);
+}
+
 template <int P, typename R>
 int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stages,
                      const double *factor, const double *factor_tilde, double fm,
@@ -330,6 +411,62 @@ int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stag
   const bool last = stage == n_stages - 1;
   const dim3 grid(op->NB), block(CB);
   hipEvent_t ev = nullptr;
+  if (O.nu == 0.0 && op->paired_items)
+    {
+      // one launch: k_stage_ark_pair
+      OpArgs Oe = O;
+      if (!last)
+        Oe.has_friction = 0;
+      StageArgs H{}, Q{};
+      H.src_h = ri_h;
+      H.src_q = ri_q;
+      H.dst = ki_h[stage + 1];
+      for (uint32_t j = 0; j < stage; ++j)
+        H.kprev[j] = ki_h[j + 1];
+      for (uint32_t j = 0; j <= stage; ++j)
+        H.factor[j] = factor[j];
+      H.stage = (int)stage;
+      H.wsol = w_h;
+      H.rk_dst = last ? w_h : r_h;
+      H.sweeps_blk = O.sweeps;
+      Q.src_h = ri_h;
+      Q.src_q = ri_q;
+      Q.src_v = ri_v;
+      Q.dst = ki_q[2 * stage + 1];
+      ArkArgs A{};
+      A.ri_h = ri_h;
+      A.ri_q = ri_q;
+      for (uint32_t j = 0; j <= stage; ++j)
+        {
+          A.kex[j] = ki_q[2 * j + 1];
+          A.kim[j] = ki_q[2 * j + 2 < 2 * n_stages ? 2 * j + 2 : 0];
+          A.factor[j] = factor[j];
+          A.factor_tilde[j] = factor_tilde ? factor_tilde[j] : 0.0;
+        }
+      A.fm = fm;
+      A.stage = (int)stage;
+      A.last = last;
+      A.wsol = w_q;
+      A.rdst = r_q;
+      A.wq_out = w_q;
+      timed_begin(op, s, &ev);
+      hipLaunchKernelGGL((k_stage_ark_pair<P, R>), grid, dim3(2 * CB), 0, s, Oe, H, Q, A);
+      timed_end(op, s, TK_STAGE, ev);
+      HIP_TRY(hipGetLastError());
+      if (std::is_same<R, float>::value && defer_depth_solve(P))
+        {
+          // FP64 depth solve of the cells the FP32 stage left near the depth floor
+          StageArgs D = H;
+          for (uint32_t j = 0; j <= stage; ++j)
+            D.kprev[j] = ki_h[j + 1];
+          timed_begin(op, s, &ev);
+          const dim3 dgrid((op->NB + DEFER_BATCHES - 1) / DEFER_BATCHES);
+          hipLaunchKernelGGL((k_depth_deferred<P>), dgrid, block, 0, s, O, D);
+          timed_end(op, s, TK_HEIGHT_JACOBI, ev);
+          HIP_TRY(hipGetLastError());
+        }
+      return OC_OK;
+    }
   timed_begin(op, s, &ev);
   hipLaunchKernelGGL((k_height_rhs<P, false, R, R>), grid, block, 0, s, O, ri_h, ri_q,
                      ki_h[stage + 1]);

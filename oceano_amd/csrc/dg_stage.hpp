@@ -1098,7 +1098,9 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
       // faces 2 and 3 for both equations (pair_exchange); the LDS copy of the batch's own
       // data staged above is the height wave's too (read after this barrier, until the
       // exchange)
-      static_assert(!VISC && MODE == 0, "paired items: inviscid explicit stage only");
+      static_assert(!VISC && (MODE == 0 || MODE == 4),
+                    "paired items: inviscid stage (explicit, or the additive stage's explicit "
+                    "momentum integrals)");
       __syncthreads();
       R A[3][ND];
 #pragma unroll
