@@ -32,6 +32,7 @@ struct ArkArgs
   const double *wsol;  // w_q
   double *rdst;        // r_q
   double *wq_out;      // w_q at the last stage
+  double *kim_out;     // this stage's stiff k vector (ki_q[2 stage + 2]; paired stage only)
 };
 
 // the stiff friction source of one cell tested against the basis: Rx, Ry (Gauss(nq)); own
@@ -363,7 +364,7 @@ __device__ __forceinline__ void discharge_ark_block(const OpArgs &O, const Stage
       const LdsVec<R> Z{lds + lane}, QX{lds + ND * CB + lane}, QY{lds + 2 * ND * CB + lane};
       R Rx[ND], Ry[ND];
       stiff_source<P, R>(O, basis<R>(P), Z, QX, QY, R(zc), batch, lane, hx * hy, Rx, Ry);
-      R *kim = reinterpret_cast<R *>(const_cast<double *>(A.kim[A.stage]));
+      R *kim = reinterpret_cast<R *>(A.kim_out);
 #pragma unroll
       for (int k = 0; k < ND; ++k)
         {
@@ -449,6 +450,7 @@ int launch_stage_ark(oc_dg *op, const OpArgs &O, uint32_t stage, uint32_t n_stag
       A.wsol = w_q;
       A.rdst = r_q;
       A.wq_out = w_q;
+      A.kim_out = last ? nullptr : ki_q[2 * stage + 2];
       timed_begin(op, s, &ev);
       hipLaunchKernelGGL((k_stage_ark_pair<P, R>), grid, dim3(2 * CB), 0, s, Oe, H, Q, A);
       timed_end(op, s, TK_STAGE, ev);
