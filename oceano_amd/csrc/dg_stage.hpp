@@ -1939,8 +1939,37 @@ __host__ __device__ constexpr int pair_xch_offset()
   return std::is_same<R, float>::value && P >= 3 ? matvec_lds_words<(P + 1) * (P + 1), 1>() : 0;
 }
 
-// the whole stage in one launch with paired work items: workgroup = one batch, wave 0 its
-// height item and wave 1 its discharge item, the faces split between them (pair_exchange).
+// LDS of a paired workgroup (words of R): the height wave's slice (the FP32 depth
+// increment's matrix-core product, then the exchange area), rounded up to 64 words,
+// followed by the discharge wave's slice (its own data, later its matrix-core product).
+// The height slice is only as large as its contents: at FP64 p = 3 a workgroup takes
+// 36 KiB, so four workgroups (two waves per SIMD) fit a CU's 160 KiB of LDS, against
+// three with two equal 24 KiB slices.
+template <int P, typename R>
+__host__ __device__ constexpr int pair_height_words()
+{
+  return (pair_xch_offset<P, R>() + pair_xch_words<P>() + 63) / 64 * 64;
+}
+
+template <int P, typename R>
+__host__ __device__ constexpr int pair_lds_words()
+{
+  return pair_height_words<P, R>() + discharge_lds_words<P, false>();
+}
+
+// Which wave of a paired workgroup runs the height item: wave item_kind(blockIdx.x), the
+// other the discharge item.  With the height item always on wave 0, the hardware's placement
+// of a workgroup's waves on a CU's SIMDs could put every height wave on some SIMDs and every
+// discharge wave (the longer item) on the others; alternating the roles along the Thue-Morse
+// sequence of the workgroup index mixes the two kinds on every SIMD.
+__device__ __forceinline__ bool pair_height_wave()
+{
+  return (int)(threadIdx.x / CB) == item_kind((int)blockIdx.x);
+}
+
+// the whole stage in one launch with paired work items: workgroup = one batch, one wave its
+// height item and the other its discharge item (pair_height_wave), the faces split between
+// them (pair_exchange).
 // Two workgroup barriers: after the discharge wave has staged the batch's own data in its
 // LDS slice (which the height wave then reads too), and in pair_exchange.  Workgroups are
 // dealt so that each XCD holds a contiguous range of batches.
@@ -1948,16 +1977,14 @@ template <int P, typename R>
 __global__ __launch_bounds__(2 * CB, (stage_min_waves<P, false, R>())) void k_stage_pair(
   OpArgs O, StageArgs H, StageArgs Q)
 {
-  __shared__ R lds[2][discharge_lds_words<P, false>()];
-  static_assert(pair_xch_offset<P, R>() + pair_xch_words<P>() <= discharge_lds_words<P, false>(),
-                "LDS of the height item too small for the paired exchange");
+  __shared__ __align__(16) R lds[pair_lds_words<P, R>()];
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
-  R *xch = lds[0] + pair_xch_offset<P, R>();
-  if (threadIdx.x < CB)
-    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch,
-                                   lds[1]);
+  R *xch = lds + pair_xch_offset<P, R>();
+  R *dlds = lds + pair_height_words<P, R>();
+  if (pair_height_wave())
+    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds), xch, dlds);
   else
-    discharge_block<P, false, 0, R, true>(O, Q, batch, lds[1], xch);
+    discharge_block<P, false, 0, R, true>(O, Q, batch, dlds, xch);
 }
 
 }  // namespace oceano

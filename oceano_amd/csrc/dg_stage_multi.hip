@@ -83,25 +83,30 @@ __global__ __launch_bounds__(CB, (multi_min_waves<R>())) void k_stage_multi(
     }
 }
 
-// paired items (k_stage_pair): workgroup = one batch of a group, wave 0 its height item,
-// wave 1 its discharge item
+// paired items (k_stage_pair): workgroup = one batch of a group, one wave its height item,
+// the other its discharge item (pair_height_wave); the LDS is laid out as k_stage_pair's at degree 4 (the height
+// slice of degree 4 is the largest, so every degree's height slice fits in front of the
+// discharge slice)
 template <int P, typename R>
 __device__ __forceinline__ void multi_pair(const OpArgs &O, const StageArgs &H,
-                                           const StageArgs &Q, int batch, R (&lds)[2][discharge_lds_words<4, false>()])
+                                           const StageArgs &Q, int batch, R *lds)
 {
-  R *xch = lds[0] + pair_xch_offset<P, R>();
-  if (threadIdx.x < CB)
-    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds[0]), xch,
-                                   lds[1]);
+  static_assert(pair_height_words<P, R>() <= pair_height_words<4, R>() &&
+                  discharge_lds_words<P, false>() <= discharge_lds_words<4, false>(),
+                "paired LDS slices of a lower degree exceed degree 4's");
+  R *xch = lds + pair_xch_offset<P, R>();
+  R *dlds = lds + pair_height_words<4, R>();
+  if (pair_height_wave())
+    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds), xch, dlds);
   else
-    discharge_block<P, false, 0, R, true>(O, Q, batch, lds[1], xch);
+    discharge_block<P, false, 0, R, true>(O, Q, batch, dlds, xch);
 }
 
 template <typename R>
 __global__ __launch_bounds__(2 * CB, (multi_min_waves<R>())) void k_stage_multi_pair(
   const MultiArgs *__restrict__ A)
 {
-  __shared__ R lds[2][discharge_lds_words<4, false>()];
+  __shared__ __align__(16) R lds[pair_lds_words<4, R>()];
   const int b = blockIdx.x;
   int g = 0;
   while (g + 1 < A->n_groups && b >= A->start[g + 1])
