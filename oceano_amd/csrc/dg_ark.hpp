@@ -376,9 +376,8 @@ __device__ __forceinline__ void discharge_ark_block(const OpArgs &O, const Stage
 }
 
 // The additive stage in one launch with paired work items (inviscid): workgroup = one batch,
-// one wave the height item (L_zeta, front and depth solve: the explicit stage's height item
-// with the additive stage's factors), the other the momentum item (discharge_ark_block;
-// roles as in k_stage_pair).  The
+// wave 0 the height item (L_zeta, front and depth solve: the explicit stage's height item
+// with the additive stage's factors), wave 1 the momentum item (discharge_ark_block).  The
 // operator arguments carry the friction switched off before the last stage (the stiff term
 // takes it; local_apply_cell_nonstiff_discharge).
 // waves per SIMD of the paired additive stage: the momentum item's FP64 modified-mass update
@@ -399,7 +398,9 @@ __global__ __launch_bounds__(2 * CB, (ark_min_waves<P, R>())) void k_stage_ark_p
   const int batch = logical_batch(blockIdx.x, O.n_batches, O.xcd_swizzle);
   R *xch = lds + pair_xch_offset<P, R>();
   R *dlds = lds + pair_height_words<P, R>();
-  if (pair_height_wave())
+  // the height item on wave 0 of every workgroup: unlike the explicit stage (pair_height_wave),
+  // alternating the roles measured slower here (mixed C2 stage 119-122 against 106-112 us)
+  if (threadIdx.x < CB)
     height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds), xch, dlds);
   else
     discharge_ark_block<P, R>(O, Q, A, batch, dlds, xch);
