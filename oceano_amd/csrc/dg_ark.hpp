@@ -383,11 +383,12 @@ __device__ __forceinline__ void discharge_ark_block(const OpArgs &O, const Stage
 // waves per SIMD of the paired additive stage: the momentum item's FP64 modified-mass update
 // needs more registers than the explicit stage's momentum item.  FP64 at p <= 3: two waves
 // (at p = 3, 256 VGPRs and 20 spill slots; measured 173 us per C2 stage against 204 us at
-// one wave per SIMD with no spills; the workgroup's 36 KiB of LDS lets four share a CU)
+// one wave per SIMD with no spills; the workgroup's 36 KiB of LDS lets four share a CU).
+// FP32 at p = 4: one wave (two would spill 279 slots)
 template <int P, typename R>
 constexpr int ark_min_waves()
 {
-  return std::is_same<R, float>::value ? (P <= 2 ? 3 : 2) : (P <= 3 ? 2 : 1);
+  return std::is_same<R, float>::value ? (P <= 2 ? 3 : P == 3 ? 2 : 1) : (P <= 3 ? 2 : 1);
 }
 
 template <int P, typename R>

@@ -6,7 +6,7 @@ import pytest
 import torch
 
 from helpers import make_pair, rel_l2, to_dev, to_host
-from oceano_amd import problems
+from oceano_amd import problems, tune
 from oceano_amd.integrator import (AdditiveRungeKuttaIntegrator, ExplicitRungeKuttaIntegrator,
                                    ark_tableau, butcher_rows)
 from oceano_amd.mesh import Mesh
@@ -64,6 +64,31 @@ def test_ark_vortex_matches_oracle(cuda, degree):
     # friction acted: the IMEX step differs from the frictionless one
     assert rel_l2(gq, _oracle_steps(make_pair(_rough_vortex(degree, 0.0))[1], h, q, dt,
                                     3)[1]) > 1e-8
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+def test_ark_vortex_other_stage_form_matches_oracle(cuda, degree):
+    """the stage form the default does not pick at this degree: the five-kernel stage at
+    p <= 3, the one-launch paired stage (k_stage_ark_pair) at p = 4 (library option
+    paired_items); FP64 against the oracle at the parity bar, FP32 integrals within 1e-6"""
+    P = _rough_vortex(degree, 0.1)
+    pair = 1 if degree == 4 else 0
+    print("[ark] vortex p=%d paired_items=%d" % (degree, pair), flush=True)
+    tune.apply_options({"paired_items": pair})
+    try:
+        op, o, h, q = make_pair(P)
+        opm, _, _, _ = make_pair(P, precision="mixed")
+    finally:
+        tune.apply_options({"paired_items": -1})
+    dt = 2e-4
+    oh, oq = _oracle_steps(o, h, q, dt, 3)
+    gh, gq = _gpu_steps(op, h, q, dt, 3)
+    mh, mq = _gpu_steps(opm, h, q, dt, 3)
+    e = {"fp64 zeta": rel_l2(gh, oh), "fp64 q": rel_l2(gq, oq),
+         "mixed zeta": rel_l2(mh, oh), "mixed q": rel_l2(mq, oq)}
+    print("  3 IMEX steps: " + " ".join("%s=%.2e" % kv for kv in e.items()))
+    assert e["fp64 zeta"] < 1e-12 and e["fp64 q"] < 1e-12
+    assert e["mixed zeta"] < 1e-6 and e["mixed q"] < 1e-6
 
 
 def test_ark_channel_matches_oracle(cuda):
