@@ -1913,22 +1913,41 @@ __device__ __forceinline__ int item_kind(int w) { return __popc((unsigned)w) & 1
 // the whole stage in one launch: work items w = (batch, kind), WPB waves (items) per
 // workgroup, each wave on its own LDS slice; workgroups are dealt so that each XCD holds a
 // contiguous range of batches
+// LDS words (of R) a height item of the unpaired stage uses: the FP32 depth increment's
+// matrix-core operands at p >= 3, rounded up to 64 words; nothing otherwise
+template <int P, typename R>
+__host__ __device__ constexpr int height_lds_words()
+{
+  return std::is_same<R, float>::value && P >= 3
+           ? (matvec_lds_words<(P + 1) * (P + 1), 1>() + 63) / 64 * 64
+           : 0;
+}
+
+// With WPB >= 2 the items of a workgroup are the two items of WPB / 2 batches (w = 2b and
+// 2b + 1 always have opposite kinds), so the workgroup's LDS holds, per batch, a height
+// slice followed by a discharge slice, each of its own size (mixed p = 4, WPB = 2: 29 KiB per
+// workgroup, five per CU, against 38 KiB and four with two discharge-sized slices).  With
+// WPB = 1 each workgroup holds one discharge-sized slice, whatever kind its item is.
 template <int P, bool VISC, typename R, int WPB>
 __global__ __launch_bounds__(CB * WPB, (stage_min_waves<P, VISC, R>())) void k_stage(
   OpArgs O, StageArgs H, StageArgs Q)
 {
-  __shared__ R lds[WPB][discharge_lds_words<P, VISC>()];
-  static_assert(P < 3 || sizeof(lds[0]) >= sizeof(float) * matvec_lds_words<(P + 1) * (P + 1), 1>(),
+  constexpr int HW = height_lds_words<P, R>(), DW = discharge_lds_words<P, VISC>();
+  __shared__ __align__(16) R lds[WPB == 1 ? DW : (WPB / 2) * (HW + DW)];
+  static_assert(P < 3 || sizeof(R) * DW >= sizeof(float) * matvec_lds_words<(P + 1) * (P + 1), 1>(),
                 "LDS of the stage too small for the matrix-core depth increment");
+  static_assert(WPB == 1 || WPB % 2 == 0, "work items per workgroup: 1 or an even number");
   const int n_items = 2 * O.n_batches;
   const int wave = WPB == 1 ? 0 : (int)(threadIdx.x / CB);
   const int w = logical_batch(blockIdx.x, (n_items + WPB - 1) / WPB, O.xcd_swizzle) * WPB + wave;
   if (w >= n_items)
     return;
-  if (item_kind(w))
-    discharge_block<P, VISC, 0, R>(O, Q, w >> 1, lds[wave]);
+  const int kind = item_kind(w);
+  R *slice = WPB == 1 ? lds : lds + (wave / 2) * (HW + DW) + (kind ? HW : 0);
+  if (kind)
+    discharge_block<P, VISC, 0, R>(O, Q, w >> 1, slice);
   else
-    height_stage_block<P, R>(O, H, w >> 1, reinterpret_cast<float *>(lds[wave]));
+    height_stage_block<P, R>(O, H, w >> 1, reinterpret_cast<float *>(slice));
 }
 
 // LDS offset (words of R) of the paired items' exchange area inside the height wave's slice:
