@@ -214,14 +214,17 @@ def main():
     ap.add_argument("--paired", type=int, default=None, choices=[0, 1],
                     help="library option paired_items (a batch's height and discharge "
                          "items in one workgroup, faces split between them)")
+    ap.add_argument("--waves-per-block", type=int, default=None, choices=[1, 2, 4],
+                    help="library option waves_per_block (work items per workgroup of the "
+                         "unpaired stage kernel)")
     ap.add_argument("--ark", action="store_true",
                     help="also time the additive (IMEX) stage on the same workload")
     ap.add_argument("--steady", action="store_true",
                     help="also run the workload to steady state (the channel's criterion: one "
                          "step changes the state by < 1e-10 relative L2) and time the whole run")
     ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
-                    help="time the work-layout options per operator and keep the fastest "
-                         "(auto: on for the ensemble)")
+                    help="time the work-layout options per operator and precision and keep "
+                         "the fastest (auto: on for every workload; off: library defaults)")
     ap.add_argument("--ensemble-launch", default="auto", choices=["auto", "streams", "single"],
                     help="several operators: one stream per operator (per-degree launches) "
                          "or one launch per stage for all of them; auto times both on "
@@ -245,6 +248,10 @@ def main():
     if args.slot_chunk is not None:
         from oceano_amd import _capi
         _capi.check(_capi.lib().oc_set_option(b"slot_chunk", args.slot_chunk), "oc_set_option")
+    if args.waves_per_block is not None:
+        from oceano_amd import _capi
+        _capi.check(_capi.lib().oc_set_option(b"waves_per_block", args.waves_per_block),
+                    "oc_set_option")
     groups, dts, degree_label, note = workload(args.workload)
     if args.dt is not None:
         dts = [args.dt] * len(groups)
@@ -252,7 +259,7 @@ def main():
     multi = len(groups) > 1
     n_steps_timed = args.steps if not multi else S * (-(-args.steps // S))
 
-    tune_on = args.autotune == "on" or (args.autotune == "auto" and multi)
+    tune_on = args.autotune != "off"
     tuned = {}
 
     def run(precision):
