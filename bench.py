@@ -597,9 +597,11 @@ def run_steady(problem, dt, precision, tol=1e-10, chunk=2000, max_steps=400000):
 
 
 def run_ark(problem, dt, precision, n_stages):
-    """The additive (IMEX) stage_3_order_2 scheme on the same workload: eager stages (five
-    kernels each: L_zeta, explicit L_q, stiff source, depth solve, modified-mass update)
-    timed between two synchronisations; DoF-stages/s and the per-kernel event times."""
+    """The additive (IMEX) stage_3_order_2 scheme on the same workload with the library's
+    default layout: eager stages (one launch each, k_stage_ark_pair, for an inviscid operator
+    with paired work items; else five kernels: L_zeta, explicit L_q, stiff source, depth
+    solve, modified-mass update) timed between two synchronisations; DoF-stages/s and the
+    per-kernel event times."""
     import torch
     from oceano_amd import problems
     from oceano_amd.integrator import AdditiveRungeKuttaIntegrator
