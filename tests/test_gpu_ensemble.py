@@ -143,12 +143,15 @@ def test_single_launch_equals_per_degree_launches(cuda, precision, degrees):
         assert torch.equal(g_h[g], e_h[g]) and torch.equal(g_q[g], e_q[g]), g
 
 
+@pytest.mark.parametrize("degree", [2, 3, 4])
 @pytest.mark.parametrize("wpb", [2, 4])
-def test_waves_per_block_does_not_change_results(cuda, wpb):
+def test_waves_per_block_does_not_change_results(cuda, wpb, degree):
     """64-cell work items per workgroup of the one-pass stage (library option
-    waves_per_block): the same steps bit for bit"""
+    waves_per_block; each batch's height and discharge items on LDS slices of their own
+    sizes): the same steps bit for bit"""
     from oceano_amd import tune
-    P = problems.vortex(degree=3, refine=1, refine_fraction=0.3, seed=5)
+    print("[wpb] degree=%d waves_per_block=%d" % (degree, wpb), flush=True)
+    P = problems.vortex(degree=degree, refine=1, refine_fraction=0.3, seed=5)
     out = []
     try:
         for w in (1, wpb):
