@@ -40,9 +40,10 @@ const char *oc_last_error(void);
 
 /* Library tuning options read by later oc_dg_create calls (no reference counterpart: the
  * reference leaves cell batching to deal.II's MatrixFree).
- *   "waves_per_block" 1 (default), 2 or 4: 64-cell work items per workgroup of the one-pass
- *                 stage kernel (64, 128 or 256 cells per workgroup).  Results do not
- *                 depend on it.
+ *   "waves_per_block" 1 (default), 2 or 4: 64-cell work items per workgroup of the
+ *                 unpaired one-pass stage kernel (with 2 or 4, the height and discharge
+ *                 items of 1 or 2 batches, each on an LDS slice of its own size).  Results
+ *                 do not depend on it.
  *   "paired_items" -1 (default: 1 at degree <= 3, 0 at degree 4), 0 or 1: the
  *                 inviscid one-pass stage runs each batch's height and discharge items as
  *                 one two-wave workgroup that splits the faces between them (each face
