@@ -401,7 +401,7 @@ __global__ __launch_bounds__(2 * CB, (ark_min_waves<P, R>())) void k_stage_ark_p
   R *dlds = lds + pair_height_words<P, R>();
   // the height item on wave 0 of every workgroup: unlike the explicit stage (pair_height_wave),
   // alternating the roles measured slower here (mixed C2 stage 119-122 against 106-112 us)
-  if (threadIdx.x < CB)
+  if (wave_in_block() == 0)
     height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds), xch, dlds);
   else
     discharge_ark_block<P, R>(O, Q, A, batch, dlds, xch);

@@ -153,6 +153,14 @@ __device__ __forceinline__ void phase_barrier()
 // registers), so the wave waits out one memory latency per load instead of one per group.
 __device__ __forceinline__ void issue_barrier() { __builtin_amdgcn_sched_barrier(0); }
 
+// index of the calling wave in its workgroup, as a wave-uniform (scalar) value: the
+// compiler cannot tell that threadIdx.x / 64 is the same on every lane, and a branch on it
+// would be compiled as divergent (both sides' live ranges overlapping, values in VGPRs)
+__device__ __forceinline__ int wave_in_block()
+{
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / CB));
+}
+
 // boundary data of the running stage (see OpArgs::bvals)
 __device__ __forceinline__ const double *stage_bvals(const OpArgs &O)
 {

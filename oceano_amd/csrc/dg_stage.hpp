@@ -1938,7 +1938,7 @@ __global__ __launch_bounds__(CB * WPB, (stage_min_waves<P, VISC, R>())) void k_s
                 "LDS of the stage too small for the matrix-core depth increment");
   static_assert(WPB == 1 || WPB % 2 == 0, "work items per workgroup: 1 or an even number");
   const int n_items = 2 * O.n_batches;
-  const int wave = WPB == 1 ? 0 : (int)(threadIdx.x / CB);
+  const int wave = WPB == 1 ? 0 : wave_in_block();
   const int w = logical_batch(blockIdx.x, (n_items + WPB - 1) / WPB, O.xcd_swizzle) * WPB + wave;
   if (w >= n_items)
     return;
@@ -1983,7 +1983,7 @@ __host__ __device__ constexpr int pair_lds_words()
 // sequence of the workgroup index mixes the two kinds on every SIMD.
 __device__ __forceinline__ bool pair_height_wave()
 {
-  return (int)(threadIdx.x / CB) == item_kind((int)blockIdx.x);
+  return wave_in_block() == item_kind((int)blockIdx.x);
 }
 
 // the whole stage in one launch with paired work items: workgroup = one batch, one wave its
