@@ -77,6 +77,10 @@ struct oc_dg
   int xcd_swizzle = 1;
   int waves_per_block = 1;  // one-pass stage kernel: 64-cell items per workgroup (1, 2, 4)
   int paired_items = 0;     // inviscid one-pass stage: k_stage_pair (faces split per batch)
+  // set by the device time loop (oc_rk) around each stage: the stage input is in its centred
+  // FP32 form (rk_in32), the next stage input is to be written so (rk_out32); honoured only
+  // by the FP32 paired stage at p <= 3, whose integrals read the input only as FP32 numbers
+  int rk_in32 = 0, rk_out32 = 0;
   // optional per-kernel event timing: kernel id -> recorded (start, stop) pairs
   int timing = 0;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;

@@ -101,6 +101,9 @@ struct StageArgs
   const double *kprev[MAXKI];// ki[1..stage] for the RK combination
   double factor[MAXKI];
   int stage;                 // number of previous k vectors = stage
+  // FP32 paired stage inside the device time loop: rk_dst receives the next stage input in
+  // its centred FP32 form instead of FP64 (c32_values, c32_centres)
+  int out32;
   const double *wsol;        // solution (w_h or w_q)
   double *rk_dst;            // r or w
   unsigned int *sweeps_blk;  // per-batch sum of Jacobi sweeps (height kernels)
@@ -159,6 +162,20 @@ __device__ __forceinline__ void issue_barrier() { __builtin_amdgcn_sched_barrier
 __device__ __forceinline__ int wave_in_block()
 {
   return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / CB));
+}
+
+// The centred FP32 form of an intermediate stage input (the FP32 stage's integrals read it
+// only so): in the storage of the FP64 vector, the values as floats at the float offset
+// vix(...) (the first half of the storage; zeta minus its cell's DoF-0 value, q as they
+// are), and for zeta each cell's centre, its DoF-0 value in FP64, after them.  The
+// integrals then see the same FP32 numbers as when they convert the FP64 vector.
+__device__ __forceinline__ const float *c32_values(const double *v)
+{
+  return reinterpret_cast<const float *>(v);
+}
+__device__ __forceinline__ const double *c32_centres(const double *v, int n_batches, int nd)
+{
+  return v + (size_t)n_batches * nd * CB / 2;
 }
 
 // boundary data of the running stage (see OpArgs::bvals)

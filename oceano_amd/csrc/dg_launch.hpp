@@ -101,7 +101,15 @@ int launch_stage(oc_dg *op, const OpArgs &O, uint32_t stage, const double *facto
       if (visc)
         hipLaunchKernelGGL((k_stage<P, true, R, 1>), dim3(items), block, 0, s, O, H, Q);
       else if (op->paired_items)
-        hipLaunchKernelGGL((k_stage_pair<P, R>), grid, dim3(2 * CB), 0, s, O, H, Q);
+        {
+          if (std::is_same<R, float>::value && P <= 3)
+            H.out32 = Q.out32 = op->rk_out32 && !last;
+          if (std::is_same<R, float>::value && P <= 3 && op->rk_in32)
+            hipLaunchKernelGGL((k_stage_pair<P, R, std::is_same<R, float>::value && P <= 3>),
+                               grid, dim3(2 * CB), 0, s, O, H, Q);
+          else
+            hipLaunchKernelGGL((k_stage_pair<P, R>), grid, dim3(2 * CB), 0, s, O, H, Q);
+        }
       else if (op->waves_per_block == 4)
         hipLaunchKernelGGL((k_stage<P, false, R, 4>), dim3((items + 3) / 4), dim3(4 * CB), 0,
                            s, O, H, Q);

@@ -113,10 +113,15 @@ int step(oc_rk *rk, double dt, double *w_h, double *w_q, double *velocity, hipSt
         v *= dt;
       const double *ri_h = st == 0 ? w_h : rk->rh[st % 2];
       const double *ri_q = st == 0 ? w_q : rk->rq[st % 2];
+      // the intermediate stage inputs live only in the integrator's r registers: the FP32
+      // paired stage may pass them in their centred FP32 form (oc_dg::rk_in32 / rk_out32)
+      rk->op->rk_in32 = st > 0;
+      rk->op->rk_out32 = st + 1 < S;
       const int rc = oc_dg_perform_stage_hydro(
         rk->op, (uint32_t)st, rk->t + rk->c[st] * dt, f.data(), (uint32_t)st + 1, ri_h, ri_q,
         velocity, rk->kh.data(), rk->kq.data(), (uint32_t)S + 1, w_h, w_q,
         rk->rh[(st + 1) % 2], rk->rq[(st + 1) % 2], 0, s);
+      rk->op->rk_in32 = rk->op->rk_out32 = 0;
       if (rc)
         return rc;
     }

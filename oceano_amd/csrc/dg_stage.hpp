@@ -230,12 +230,17 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                                            const double *__restrict__ src_h,
                                            const double *__restrict__ src_q,
                                            R (&Racc)[NC][(P + 1) * (P + 1)], InnerFn inner,
-                                           BdryFn bdry)
+                                           BdryFn bdry, bool in32 = false)
 {
   constexpr int N1 = P + 1, ND = N1 * N1, NQ = nq_of(P), NG = P + 2;
   constexpr bool COMPACT = std::is_same<R, float>::value;
   constexpr int NS = COMPACT ? 1 : 2;  // subfaces integrated in the face loop
   const R zcr = R(zc);
+  // stage input in its centred FP32 form (FP32 stage only; c32_values, c32_centres)
+  const bool c32 = centred<R>() && in32;
+  const float *__restrict__ fh = c32_values(src_h);
+  const float *__restrict__ fq = c32_values(src_q);
+  const double *__restrict__ zcen = c32_centres(src_h, O.n_batches, ND);
   // the face tables of all four faces in one group of loads: each face then waits for one
   // memory latency (its neighbour's traces) instead of three dependent ones
   int infos[4] = {}, nbrs[4][NS] = {}, fids[4][NS] = {};
@@ -319,20 +324,38 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
           // neighbour traces of the subfaces in one group of loads (the second only on the
           // coarse side, and only where it is integrated here): one memory latency per face
           phase_barrier();
-          double zg[NS][N1], xg[NS][N1], yg[NS][N1], zc2[NS];
+          // zsub: what is subtracted from zg to centre it (0 when it comes centred)
+          double zg[NS][N1], xg[NS][N1], yg[NS][N1], zc2[NS], zsub[NS];
 #pragma unroll
           for (int s = 0; s < NS; ++s)
             if (s == 0 || kind == FK_COARSE)
               {
                 const int b2 = nbrs[f][s] / CB, l2 = nbrs[f][s] % CB;
-                zc2[s] = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
-#pragma unroll
-                for (int k = 0; k < N1; ++k)
+                if (c32)
                   {
-                    const int d = face_dof<N1>(f ^ 1, k);
-                    zg[s][k] = src_h[vix(b2, 1, 0, ND, d, l2)];
-                    xg[s][k] = src_q[vix(b2, 2, 0, ND, d, l2)];
-                    yg[s][k] = src_q[vix(b2, 2, 1, ND, d, l2)];
+                    zc2[s] = zcen[(size_t)b2 * CB + l2];
+                    zsub[s] = 0.0;
+#pragma unroll
+                    for (int k = 0; k < N1; ++k)
+                      {
+                        const int d = face_dof<N1>(f ^ 1, k);
+                        zg[s][k] = fh[vix(b2, 1, 0, ND, d, l2)];
+                        xg[s][k] = fq[vix(b2, 2, 0, ND, d, l2)];
+                        yg[s][k] = fq[vix(b2, 2, 1, ND, d, l2)];
+                      }
+                  }
+                else
+                  {
+                    zc2[s] = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
+                    zsub[s] = zc2[s];
+#pragma unroll
+                    for (int k = 0; k < N1; ++k)
+                      {
+                        const int d = face_dof<N1>(f ^ 1, k);
+                        zg[s][k] = src_h[vix(b2, 1, 0, ND, d, l2)];
+                        xg[s][k] = src_q[vix(b2, 2, 0, ND, d, l2)];
+                        yg[s][k] = src_q[vix(b2, 2, 1, ND, d, l2)];
+                      }
                   }
               }
           issue_barrier();
@@ -352,7 +375,7 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
                   zo_s[k] = zo_l[k];
                   xo_s[k] = xo_l[k];
                   yo_s[k] = yo_l[k];
-                  zn_s[k] = R(zg[s][k] - zc2[s]);
+                  zn_s[k] = R(zg[s][k] - zsub[s]);
                   xn_s[k] = R(xg[s][k]);
                   yn_s[k] = R(yg[s][k]);
                 }
@@ -424,29 +447,51 @@ __device__ __forceinline__ void face_terms(const OpArgs &O, const Basis<R> &T, i
           const int fi = O.fid[(fo * 2 + 1) * CB + owner];
           const double hsz = O.hxy[((size_t)batch * 2 + (f < 2 ? 1 : 0)) * CB + owner];
           const int b2 = nb / CB, l2 = nb % CB;
-          double zt[N1], xt[N1], yt[N1], zg[N1], xg[N1], yg[N1];
-          const double zco = centred<R>() ? src_h[vix(batch, 1, 0, ND, 0, owner)] : 0.0;
-          const double zc2 = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
-#pragma unroll
-          for (int k = 0; k < N1; ++k)
+          double zt[N1], xt[N1], yt[N1], zg[N1], xg[N1], yg[N1], zco, zc2, osub, nsub2;
+          if (c32)
             {
-              const int d = face_dof<N1>(f, k), e = face_dof<N1>(f ^ 1, k);
-              zt[k] = src_h[vix(batch, 1, 0, ND, d, owner)];
-              xt[k] = src_q[vix(batch, 2, 0, ND, d, owner)];
-              yt[k] = src_q[vix(batch, 2, 1, ND, d, owner)];
-              zg[k] = src_h[vix(b2, 1, 0, ND, e, l2)];
-              xg[k] = src_q[vix(b2, 2, 0, ND, e, l2)];
-              yg[k] = src_q[vix(b2, 2, 1, ND, e, l2)];
+              zco = zcen[(size_t)batch * CB + owner];
+              zc2 = zcen[(size_t)b2 * CB + l2];
+              osub = nsub2 = 0.0;
+#pragma unroll
+              for (int k = 0; k < N1; ++k)
+                {
+                  const int d = face_dof<N1>(f, k), e = face_dof<N1>(f ^ 1, k);
+                  zt[k] = fh[vix(batch, 1, 0, ND, d, owner)];
+                  xt[k] = fq[vix(batch, 2, 0, ND, d, owner)];
+                  yt[k] = fq[vix(batch, 2, 1, ND, d, owner)];
+                  zg[k] = fh[vix(b2, 1, 0, ND, e, l2)];
+                  xg[k] = fq[vix(b2, 2, 0, ND, e, l2)];
+                  yg[k] = fq[vix(b2, 2, 1, ND, e, l2)];
+                }
+            }
+          else
+            {
+              zco = centred<R>() ? src_h[vix(batch, 1, 0, ND, 0, owner)] : 0.0;
+              zc2 = centred<R>() ? src_h[vix(b2, 1, 0, ND, 0, l2)] : 0.0;
+              osub = zco;
+              nsub2 = zc2;
+#pragma unroll
+              for (int k = 0; k < N1; ++k)
+                {
+                  const int d = face_dof<N1>(f, k), e = face_dof<N1>(f ^ 1, k);
+                  zt[k] = src_h[vix(batch, 1, 0, ND, d, owner)];
+                  xt[k] = src_q[vix(batch, 2, 0, ND, d, owner)];
+                  yt[k] = src_q[vix(batch, 2, 1, ND, d, owner)];
+                  zg[k] = src_h[vix(b2, 1, 0, ND, e, l2)];
+                  xg[k] = src_q[vix(b2, 2, 0, ND, e, l2)];
+                  yg[k] = src_q[vix(b2, 2, 1, ND, e, l2)];
+                }
             }
           issue_barrier();
           R zo_s[N1], xo_s[N1], yo_s[N1], zn_s[N1], xn_s[N1], yn_s[N1];
 #pragma unroll
           for (int k = 0; k < N1; ++k)
             {
-              zo_s[k] = R(zt[k] - zco);
+              zo_s[k] = R(zt[k] - osub);
               xo_s[k] = R(xt[k]);
               yo_s[k] = R(yt[k]);
-              zn_s[k] = R(zg[k] - zc2);
+              zn_s[k] = R(zg[k] - nsub2);
               xn_s[k] = R(xg[k]);
               yn_s[k] = R(yg[k]);
             }
@@ -1028,7 +1073,7 @@ __device__ __forceinline__ void discharge_cell_term(const OpArgs &O, const Basis
     }
 }
 
-template <int P, bool VISC, int MODE, typename R, bool PAIR = false>
+template <int P, bool VISC, int MODE, typename R, bool PAIR = false, bool IN32 = false>
 __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs &S, int batch,
                                                 R *lds, R *xch = nullptr)
 {
@@ -1045,25 +1090,49 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   // stage the batch's own data: contiguous, coalesced; each lane later reads only its own
   // column, so no barrier is needed
   double zc;
-  {
-    double zg[ND], xg[ND], yg[ND];
+  if constexpr (PAIR && centred<R>() && IN32)
+    {
+      // the stage input in its centred FP32 form: the values go to LDS as they are
+      const float *fh = c32_values(src_h) + (size_t)batch * ND * CB;
+      const float *fq = c32_values(src_q) + (size_t)batch * 2 * ND * CB;
+      float zg[ND], xg[ND], yg[ND];
+      zc = c32_centres(src_h, O.n_batches, ND)[(size_t)batch * CB + lane];
 #pragma unroll
-    for (int k = 0; k < ND; ++k)
-      {
-        zg[k] = gh[k * CB + lane];
-        xg[k] = gq[k * CB + lane];
-        yg[k] = gq[(ND + k) * CB + lane];
-      }
-    issue_barrier();
-    zc = centred<R>() ? zg[0] : 0.0;
+      for (int k = 0; k < ND; ++k)
+        {
+          zg[k] = fh[k * CB + lane];
+          xg[k] = fq[k * CB + lane];
+          yg[k] = fq[(ND + k) * CB + lane];
+        }
+      issue_barrier();
 #pragma unroll
-    for (int k = 0; k < ND; ++k)
-      {
-        lds[k * CB + lane] = R(zg[k] - zc);
-        lds[(ND + k) * CB + lane] = R(xg[k]);
-        lds[(2 * ND + k) * CB + lane] = R(yg[k]);
-      }
-  }
+      for (int k = 0; k < ND; ++k)
+        {
+          lds[k * CB + lane] = R(zg[k]);
+          lds[(ND + k) * CB + lane] = R(xg[k]);
+          lds[(2 * ND + k) * CB + lane] = R(yg[k]);
+        }
+    }
+  else
+    {
+      double zg[ND], xg[ND], yg[ND];
+#pragma unroll
+      for (int k = 0; k < ND; ++k)
+        {
+          zg[k] = gh[k * CB + lane];
+          xg[k] = gq[k * CB + lane];
+          yg[k] = gq[(ND + k) * CB + lane];
+        }
+      issue_barrier();
+      zc = centred<R>() ? zg[0] : 0.0;
+#pragma unroll
+      for (int k = 0; k < ND; ++k)
+        {
+          lds[k * CB + lane] = R(zg[k] - zc);
+          lds[(ND + k) * CB + lane] = R(xg[k]);
+          lds[(2 * ND + k) * CB + lane] = R(yg[k]);
+        }
+    }
   if (VISC)
     {
       const double *gv = S.src_v + (size_t)batch * 2 * ND * CB;
@@ -1119,7 +1188,7 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
           phase_barrier();
           const AllFaces<R> fn{physics<R>(O)};
           face_terms<P, 3, R, 0xC>(O, T, batch, lane, hx, hy, Z, QX, QY, zc, src_h, src_q, A,
-                                   fn, fn);
+                                   fn, fn, IN32);
         }
       pair_exchange<P, R, 1>(A, xch);
 #pragma unroll
@@ -1225,12 +1294,24 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
                 acc[i] += fj * t[i];
             }
           const double fs = S.factor[S.stage];
-#pragma unroll
-          for (int i = 0; i < CH; ++i)
+          if (centred<R>() && S.out32)
             {
-              const int c = (ch * CH + i) / ND, k = (ch * CH + i) % ND;
-              S.rk_dst[base + (size_t)i * CB] = acc[i] + fs * double(Racc[c][k]);
+              // the next stage input in its centred FP32 form (q: the values as they are)
+              float *r32 = reinterpret_cast<float *>(S.rk_dst);
+#pragma unroll
+              for (int i = 0; i < CH; ++i)
+                {
+                  const int c = (ch * CH + i) / ND, k = (ch * CH + i) % ND;
+                  r32[base + (size_t)i * CB] = float(acc[i] + fs * double(Racc[c][k]));
+                }
             }
+          else
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+              {
+                const int c = (ch * CH + i) / ND, k = (ch * CH + i) % ND;
+                S.rk_dst[base + (size_t)i * CB] = acc[i] + fs * double(Racc[c][k]);
+              }
         }
     }
 }
@@ -1760,7 +1841,7 @@ __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 
 // L_zeta, front and depth solve of one batch of cells in one pass: L_zeta stays in
 // registers for the front (it is also written out as ki_h[stage+1] for later stages)
-template <int P, typename R, bool PAIR = false>
+template <int P, typename R, bool PAIR = false, bool IN32 = false>
 __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageArgs &S,
                                                    int batch, float *lds, R *xch = nullptr,
                                                    const R *own = nullptr)
@@ -1786,7 +1867,10 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
           A[c][i] = 0;
       const double hx = O.hxy[((size_t)batch * 2) * CB + lane];
       const double hy = O.hxy[((size_t)batch * 2 + 1) * CB + lane];
-      const double zc = centred<R>() ? S.src_h[vix(batch, 1, 0, ND, 0, lane)] : 0.0;
+      const double zc =
+        !centred<R>() ? 0.0
+        : IN32        ? c32_centres(S.src_h, O.n_batches, ND)[(size_t)batch * CB + lane]
+                      : S.src_h[vix(batch, 1, 0, ND, 0, lane)];
       __syncthreads();
       if (active)
         {
@@ -1796,7 +1880,7 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
           phase_barrier();
           const AllFaces<R> fn{physics<R>(O)};
           face_terms<P, 3, R, 0x3>(O, basis<R>(P), batch, lane, R(hx), R(hy), Zo, QX, QY, zc,
-                                   S.src_h, S.src_q, A, fn, fn);
+                                   S.src_h, S.src_q, A, fn, fn, IN32);
         }
       pair_exchange<P, R, 0>(A, xch);
 #pragma unroll
@@ -1884,9 +1968,20 @@ __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageA
               S.scratch[vix(batch, 1, 0, ND, i, lane)] = F[i];
           k = jacobi_solve<P>(O, T, M, batch, lane, area, F, Z);
         }
+      if (centred<R>() && S.out32)
+        {
+          // the next stage input in its centred FP32 form: zeta minus its DoF-0 value, and
+          // that value among the centres
+          float *r32 = reinterpret_cast<float *>(S.rk_dst);
 #pragma unroll
-      for (int i = 0; i < ND; ++i)
-        S.rk_dst[vix(batch, 1, 0, ND, i, lane)] = Z[i];
+          for (int i = 0; i < ND; ++i)
+            r32[vix(batch, 1, 0, ND, i, lane)] = float(Z[i] - Z[0]);
+          S.rk_dst[(size_t)O.n_batches * ND * CB / 2 + (size_t)batch * CB + lane] = Z[0];
+        }
+      else
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          S.rk_dst[vix(batch, 1, 0, ND, i, lane)] = Z[i];
     }
   store_sweeps(S.sweeps_blk, batch, k);
 }
@@ -1992,7 +2087,10 @@ __device__ __forceinline__ bool pair_height_wave()
 // Two workgroup barriers: after the discharge wave has staged the batch's own data in its
 // LDS slice (which the height wave then reads too), and in pair_exchange.  Workgroups are
 // dealt so that each XCD holds a contiguous range of batches.
-template <int P, typename R>
+// IN32: the stage input comes in its centred FP32 form (c32_values, c32_centres; the device
+// time loop's intermediate stages); a compile-time variant, so that neither form's kernel
+// carries the other's loads
+template <int P, typename R, bool IN32 = false>
 __global__ __launch_bounds__(2 * CB, (stage_min_waves<P, false, R>())) void k_stage_pair(
   OpArgs O, StageArgs H, StageArgs Q)
 {
@@ -2001,9 +2099,10 @@ __global__ __launch_bounds__(2 * CB, (stage_min_waves<P, false, R>())) void k_st
   R *xch = lds + pair_xch_offset<P, R>();
   R *dlds = lds + pair_height_words<P, R>();
   if (pair_height_wave())
-    height_stage_block<P, R, true>(O, H, batch, reinterpret_cast<float *>(lds), xch, dlds);
+    height_stage_block<P, R, true, IN32>(O, H, batch, reinterpret_cast<float *>(lds), xch,
+                                         dlds);
   else
-    discharge_block<P, false, 0, R, true>(O, Q, batch, dlds, xch);
+    discharge_block<P, false, 0, R, true, IN32>(O, Q, batch, dlds, xch);
 }
 
 }  // namespace oceano

@@ -61,13 +61,17 @@ def _python_steps(op, rows, w_h, w_q, n):
                                    w_h, w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
 
 
+@pytest.mark.parametrize("degree", [1, 2, 3])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_device_integrator_and_graph_replay(cuda, precision):
-    """oc_rk_step equals the stage-by-stage loop bitwise; a captured step replayed three
-    times, with eager work and synchronisations between the replays, equals three eager
-    steps bitwise after every replay; and the FP64 result matches the oracle's RK4."""
-    P = problems.vortex(degree=3, refine=1, refine_fraction=0.3, seed=9)
-    print("[timeloop] integrator %s cells=%d" % (precision, P.mesh.n_cells), flush=True)
+def test_device_integrator_and_graph_replay(cuda, precision, degree):
+    """oc_rk_step equals the stage-by-stage loop bitwise (in mixed precision the loop's
+    intermediate stage inputs travel in their centred FP32 form, the stage-by-stage loop's
+    in FP64); a captured step replayed three times, with eager work and synchronisations
+    between the replays, equals three eager steps bitwise after every replay; and the FP64
+    result matches the oracle's RK4."""
+    P = problems.vortex(degree=degree, refine=1, refine_fraction=0.3, seed=9)
+    print("[timeloop] integrator %s p=%d cells=%d" % (precision, degree, P.mesh.n_cells),
+          flush=True)
     op, o, h, q = make_pair(P, precision=precision)
     dt = 2e-4
     rows, _ = butcher_rows("stage_4_order_4", dt)
