@@ -414,7 +414,11 @@ int oc_rk_step_multi(oc_rk *rk, const double *dt, double *const *w_h, double *co
 int oc_rk_capture_multi(oc_rk *rk, int32_t n_steps, const double *dt, double *const *w_h,
                         double *const *w_q, void *stream);
 /* device pointers of the integrator's registers (arrays of S+1 and 2 entries; any may be
- * null) */
+ * null).  On an OC_PRECISION_MIXED operator the k registers hold FP32 numbers (as every
+ * mixed k vector does), and with paired work items at degree <= 3 the r registers hold the
+ * intermediate stage inputs in centred FP32 form: the values as floats in the first half of
+ * each register (zeta minus its cell's DoF-0 value, q as is), then, in the zeta register,
+ * each cell's DoF-0 value as a double (the FP32 stage reads nothing else of them). */
 int oc_rk_registers(oc_rk *rk, double **ki_h, double **ki_q, double **r_h, double **r_q);
 
 /* Count non-finite entries of a device vector of `n` doubles into *count (host).  The
