@@ -59,6 +59,66 @@ def _python_steps(op, rows, w_h, w_q, n):
             ri = [w_h, w_q] if s == 0 else [r_h[s % 2], r_q[s % 2]]
             op.perform_stage_hydro(s, 0.0, rows[s][:s + 1], ri + [None, None], ki_h, ki_q,
                                    w_h, w_q, r_h[(s + 1) % 2], r_q[(s + 1) % 2])
+    return r_h, r_q
+
+
+def _device_to_host(ptr, n_doubles):
+    """n_doubles of device memory at ptr (a library-owned buffer) into a numpy array"""
+    import ctypes as C
+    import glob
+    import os
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:  # the HIP runtime torch ships with
+        hip = C.CDLL(glob.glob(os.path.join(os.path.dirname(torch.__file__), "lib",
+                                            "libamdhip64.so*"))[0])
+    hip.hipMemcpy.restype = C.c_int
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    out = np.empty(n_doubles, np.float64)
+    torch.cuda.synchronize()
+    assert hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0  # device to host
+    return out
+
+
+def test_mixed_loop_registers_hold_centred_fp32_stage_inputs(cuda):
+    """The layout oc_rk_registers documents for a mixed operator with paired work items:
+    after one step, the r registers that fed the last two stages decode (floats of the
+    first half, plus each cell's FP64 centre after them for zeta) to those stage inputs of
+    the stage-by-stage loop over FP64 vectors, within FP32 rounding of the centred values."""
+    import ctypes as C
+    from oceano_amd import _capi
+    P = problems.vortex(degree=3, refine=1, refine_fraction=0.3, seed=9)
+    op, _, h, q = make_pair(P, precision="mixed")
+    dt = 2e-4
+    rows, _ = butcher_rows("stage_4_order_4", dt)
+    S = len(rows)
+    r_h, r_q = _python_steps(op, rows, to_dev(op, h, 1), to_dev(op, q, 2), 1)
+    rk = DeviceIntegrator(op, "stage_4_order_4")
+    b_h, b_q = to_dev(op, h, 1), to_dev(op, q, 2)
+    rk.step(dt, b_h, b_q)
+    kh, kq = (C.c_void_p * (S + 1))(), (C.c_void_p * (S + 1))()
+    rh, rq = (C.c_void_p * 2)(), (C.c_void_p * 2)()
+    _capi.check(_capi.lib().oc_rk_registers(rk._h, kh, kq, rh, rq), "oc_rk_registers")
+    nd, NB = op.dofs_per_cell, op.n_cells_padded // 64
+    live = (np.arange(NB * 64) < op.n_cells).reshape(NB, 64)
+    for j in range(2):
+        bh = _device_to_host(rh[j], NB * nd * 64)
+        bq = _device_to_host(rq[j], NB * 2 * nd * 64)
+        vals = bh.view(np.float32)[:NB * nd * 64].reshape(NB, nd, 64).astype(np.float64)
+        cen = bh[NB * nd * 64 // 2:NB * nd * 64 // 2 + NB * 64].reshape(NB, 64)
+        ref = r_h[j].cpu().numpy().reshape(NB, nd, 64)
+        # float(zeta - centre) to half an FP32 ulp, the sum back in FP64 to half an FP64 ulp
+        dz = np.abs(vals + cen[:, None, :] - ref)
+        bound = 2.0 ** -24 * np.abs(ref - cen[:, None, :]) + 2.0 ** -52 * np.abs(ref) + 1e-300
+        mz = np.broadcast_to(live[:, None, :], dz.shape)
+        vq = bq.view(np.float32)[:NB * 2 * nd * 64].reshape(NB, 2, nd, 64).astype(np.float64)
+        refq = r_q[j].cpu().numpy().reshape(NB, 2, nd, 64)
+        dq = np.abs(vq - refq)
+        mq = np.broadcast_to(live[:, None, None, :], dq.shape)
+        print("  register %d: max |zeta err| / bound %.3f, max |q err| / bound %.3f"
+              % (j, (dz / bound)[mz].max(), (dq / (2.0 ** -24 * np.abs(refq) + 1e-300))[mq].max()))
+        assert np.all((dz <= bound)[mz])
+        assert np.all((dq <= 2.0 ** -24 * np.abs(refq) + 1e-300)[mq])
 
 
 @pytest.mark.parametrize("degree", [1, 2, 3])
