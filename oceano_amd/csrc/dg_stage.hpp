@@ -1073,6 +1073,8 @@ __device__ __forceinline__ void discharge_cell_term(const OpArgs &O, const Basis
     }
 }
 
+// IN32 (FP32 paired stage): the stage input S.src_h, S.src_q comes in its centred FP32 form
+// (c32_values, c32_centres); S.out32: the RK update writes the next stage input so
 template <int P, bool VISC, int MODE, typename R, bool PAIR = false, bool IN32 = false>
 __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs &S, int batch,
                                                 R *lds, R *xch = nullptr)
@@ -1840,7 +1842,8 @@ __global__ __launch_bounds__(CB) void k_depth_deferred(OpArgs O, StageArgs S)
 }
 
 // L_zeta, front and depth solve of one batch of cells in one pass: L_zeta stays in
-// registers for the front (it is also written out as ki_h[stage+1] for later stages)
+// registers for the front (it is also written out as ki_h[stage+1] for later stages).
+// IN32 / S.out32 as for discharge_block (the zeta register's centres included)
 template <int P, typename R, bool PAIR = false, bool IN32 = false>
 __device__ __forceinline__ void height_stage_block(const OpArgs &O, const StageArgs &S,
                                                    int batch, float *lds, R *xch = nullptr,
