@@ -112,6 +112,23 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def loop_stage_bytes(op, stat, n_stages, k_bytes, c32):
+    """Compulsory HBM bytes per stage of the device time loop, averaged over a step: as
+    algorithmic_bytes' one-pass stage, except that with c32 (the mixed paired stage at
+    p <= 3) the intermediate stage inputs are read and written in centred FP32 form (half
+    the bytes of the FP64 vector plus one FP64 centre per cell)."""
+    n, nd = op.n_cells, op.dofs_per_cell
+    v = 24 * nd * n
+    half = v / 2 + 8 * n
+    tot = 0.0
+    for s in range(n_stages):
+        full = algorithmic_bytes(op, s, stat, k_bytes)["stage"]
+        if c32:
+            full -= (v - half) * ((s > 0) + (s + 1 < n_stages))
+        tot += full
+    return tot / n_stages
+
+
 def algorithmic_bytes(op, stage, stat, k_bytes=8):
     """Compulsory HBM bytes of one stage (FP64 state vectors, k vectors of k_bytes per
     value -- 4 on the mixed path, whose k values are FP32 numbers --, int32 topology),
@@ -480,6 +497,23 @@ def main():
     achieved = head["kernels"][dom]["alg_GBs"]
     traffic_tab, traffic_src = measured_traffic(args.workload)
     traffic = traffic_tab.get(head_p) if dom == "stage" else None
+    loop_roof = None
+    if not multi:
+        # the timed region is the device time loop: its stages' own compulsory bytes over its
+        # time per stage (the FP32 paired stage at p <= 3 passes centred FP32 stage inputs)
+        g0 = head["groups"][0]
+        opts = tuned.get(head_p, [{}])[0].get("options", {}) if tune_on else {}
+        paired = opts.get("paired_items", -1)
+        paired = (g0["P"].degree <= 3) if paired == -1 else bool(paired)
+        c32 = (head_p == "mixed" and paired and g0["P"].degree <= 3
+               and g0["P"].viscosity == 0.0)
+        lb = loop_stage_bytes(g0["op"], g0["static_bytes"], S,
+                              4 if head_p == "mixed" else 8, c32)
+        lach = lb / (head["elapsed"] / n_steps_timed) / 1e9
+        loop_roof = {"achieved": lach, "frac": lach / HBM_PEAK_GBS,
+                     "alg_bytes_per_stage": lb, "us_per_stage":
+                     head["elapsed"] / n_steps_timed * 1e6,
+                     "stage_inputs": "centred FP32" if c32 else "FP64"}
     line = {
         "metric": "DoF-RK-stages/s on 1 GPU, mixed vs FP64 at matched L2 error; % HBM peak",
         "value": value,
@@ -505,7 +539,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
                      "traffic": traffic,
                      "traffic_source": traffic_src if traffic is not None else None,
-                     "alg_bytes_per_launch": head["kernels"][dom]["alg_bytes"]},
+                     "alg_bytes_per_launch": head["kernels"][dom]["alg_bytes"],
+                     "loop": loop_roof},
         "timing": "HIP events on the replay stream around the timed region (one captured "
                   "graph of the timed steps per operator); host clock beside it",
     }
