@@ -1094,26 +1094,34 @@ __device__ __forceinline__ void discharge_block(const OpArgs &O, const StageArgs
   double zc;
   if constexpr (PAIR && centred<R>() && IN32)
     {
-      // the stage input in its centred FP32 form: the values go to LDS as they are
-      const float *fh = c32_values(src_h) + (size_t)batch * ND * CB;
-      const float *fq = c32_values(src_q) + (size_t)batch * 2 * ND * CB;
-      float zg[ND], xg[ND], yg[ND];
+      // the stage input in its centred FP32 form: the batch's zeta and q blocks are, word for
+      // word, the LDS copy's layout, so they go there by direct-to-LDS loads (16 bytes per
+      // lane, 1 KiB per instruction; a 256-byte tail by 4-byte loads), no registers held
+      typedef __attribute__((address_space(1))) void gvoid;
+      typedef __attribute__((address_space(3))) void lvoid;
+      const char *gz = reinterpret_cast<const char *>(c32_values(src_h) + (size_t)batch * ND * CB);
+      const char *gqq =
+        reinterpret_cast<const char *>(c32_values(src_q) + (size_t)batch * 2 * ND * CB);
+      char *lz = reinterpret_cast<char *>(lds);
+      constexpr int ZB = ND * CB * 4;  // bytes of the zeta block; q's is twice that
+#pragma unroll
+      for (int o = 0; o + 1024 <= ZB; o += 1024)
+        __builtin_amdgcn_global_load_lds((gvoid *)(gz + o + lane * 16), (lvoid *)(lz + o), 16, 0,
+                                         0);
+#pragma unroll
+      for (int o = ZB / 1024 * 1024; o < ZB; o += 256)
+        __builtin_amdgcn_global_load_lds((gvoid *)(gz + o + lane * 4), (lvoid *)(lz + o), 4, 0, 0);
+#pragma unroll
+      for (int o = 0; o + 1024 <= 2 * ZB; o += 1024)
+        __builtin_amdgcn_global_load_lds((gvoid *)(gqq + o + lane * 16), (lvoid *)(lz + ZB + o),
+                                         16, 0, 0);
+#pragma unroll
+      for (int o = 2 * ZB / 1024 * 1024; o < 2 * ZB; o += 256)
+        __builtin_amdgcn_global_load_lds((gvoid *)(gqq + o + lane * 4), (lvoid *)(lz + ZB + o), 4,
+                                         0, 0);
       zc = c32_centres(src_h, O.n_batches, ND)[(size_t)batch * CB + lane];
-#pragma unroll
-      for (int k = 0; k < ND; ++k)
-        {
-          zg[k] = fh[k * CB + lane];
-          xg[k] = fq[k * CB + lane];
-          yg[k] = fq[(ND + k) * CB + lane];
-        }
-      issue_barrier();
-#pragma unroll
-      for (int k = 0; k < ND; ++k)
-        {
-          lds[k * CB + lane] = R(zg[k]);
-          lds[(ND + k) * CB + lane] = R(xg[k]);
-          lds[(2 * ND + k) * CB + lane] = R(yg[k]);
-        }
+      // the copy is complete before the workgroup barrier that follows the staging
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
   else
     {
