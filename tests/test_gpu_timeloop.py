@@ -121,17 +121,19 @@ def test_mixed_loop_registers_hold_centred_fp32_stage_inputs(cuda):
         assert np.all((dq <= 2.0 ** -24 * np.abs(refq) + 1e-300)[mq])
 
 
+@pytest.mark.parametrize("periodic", [True, False])
 @pytest.mark.parametrize("degree", [1, 2, 3])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_device_integrator_and_graph_replay(cuda, precision, degree):
+def test_device_integrator_and_graph_replay(cuda, precision, degree, periodic):
     """oc_rk_step equals the stage-by-stage loop bitwise (in mixed precision the loop's
     intermediate stage inputs travel in their centred FP32 form, the stage-by-stage loop's
     in FP64); a captured step replayed three times, with eager work and synchronisations
     between the replays, equals three eager steps bitwise after every replay; and the FP64
     result matches the oracle's RK4."""
-    P = problems.vortex(degree=degree, refine=1, refine_fraction=0.3, seed=9)
-    print("[timeloop] integrator %s p=%d cells=%d" % (precision, degree, P.mesh.n_cells),
-          flush=True)
+    P = problems.vortex(degree=degree, refine=1, refine_fraction=0.3, seed=9,
+                        periodic=periodic)
+    print("[timeloop] integrator %s p=%d periodic=%d cells=%d"
+          % (precision, degree, periodic, P.mesh.n_cells), flush=True)
     op, o, h, q = make_pair(P, precision=precision)
     dt = 2e-4
     rows, _ = butcher_rows("stage_4_order_4", dt)
